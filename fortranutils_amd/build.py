@@ -1,0 +1,61 @@
+"""Build liblmgx.so (HIP, gfx950) in-tree, and the Fortran drop-in shim when flang is present."""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+HIP_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
+    "-Wno-unused-value", "-Wno-unused-result",
+    # packed FP32 arithmetic is kept off on these machines
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build_lib(verbose: bool = False) -> str:
+    src = os.path.join(HERE, "csrc", "lmgx.hip")
+    out = os.path.join(HERE, "liblmgx.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-o", tmp, src]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
+def build_shim(verbose: bool = False) -> str | None:
+    """liblmg_shim.so: Fortran ``lmg_solve`` with the reference's 12-argument signature."""
+    if not os.path.exists(FLANG):
+        return None
+    src = os.path.join(HERE, "fortran", "lmg_solve_shim.f90")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(HERE, "liblmg_shim.so")
+    moddir = os.path.join(HERE, "fortran", "_mod")
+    os.makedirs(moddir, exist_ok=True)
+    cmd = [FLANG, "-O2", "-fPIC", "-shared", "-fdefault-real-8", "-module-dir", moddir,
+           "-o", out, src, "-L" + HERE, "-llmgx", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
+    if verbose:
+        print("built", out)
+    return out
+
+
+def build(verbose: bool = False) -> None:
+    build_lib(verbose)
+    build_shim(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,846 @@
+// liblmgx: MI355X-native pressure-Poisson multigrid for LESter's multiblock layout.
+// One HIP stream per handle; every block of a level is processed by one launch driven by the
+// device block tables (LevBlk); no host round trip inside a V-cycle except the per-cycle norm
+// read of the reference-order path.
+#include "lmgx_common.h"
+#include "../../include/lmgx.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels_exact.inc"
+#include "kernels_fast.inc"
+
+// ---------------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+extern "C" const char *lmgx_last_error(void) { return g_err.c_str(); }
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(LMGX_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+#define CHK(expr)                                                                                 \
+    do {                                                                                          \
+        int rc_ = (expr);                                                                         \
+        if (rc_ != 0) return rc_;                                                                 \
+    } while (0)
+
+static inline unsigned nblocks(long n) { return (unsigned)((n + LMGX_WG - 1) / LMGX_WG); }
+
+// ---------------------------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------------------------
+struct LevelDev {
+    std::vector<LevBlk> h_lb;
+    LevBlk *d_lb = nullptr;
+    long tiles_int = 0, tiles_all = 0, tiles_rb = 0, nlines = 0;
+    int smax = 0;                       // largest hyperplane index i+j+k (1-based)
+    std::vector<Face> h_fc;
+    Face *d_fc = nullptr;
+    int nfaces = 0;
+    long nel = 0;                       // swapped elements (all faces)
+    long npts_int = 0;                  // interior cells
+    long nsplit = 0;                    // slots per colour array in the split layout
+};
+
+struct FastBufs;  // split-layout storage of the fast path (kernels_fast.inc)
+
+struct lmgx_handle {
+    int nblk = 0, nlev = 0;
+    int nijkp1[LMGX_MAXLEV][LMGX_MAXB][3];
+    long off[LMGX_MAXLEV][LMGX_MAXB];
+    double npts[LMGX_MAXLEV];
+    int bc[LMGX_MAXB][6], bl2bl[LMGX_MAXB][6], bl2side[LMGX_MAXB][6];
+    long ntot = 0, n1 = 0, n2d = 0, n1d = 0, nxy = 0, nz = 0;
+    long off2d[LMGX_MAXLEV][LMGX_MAXB], off1d[LMGX_MAXLEV][LMGX_MAXB];
+    long offxy[LMGX_MAXLEV][LMGX_MAXB], offz[LMGX_MAXLEV][LMGX_MAXB];
+    LevelDev lev[LMGX_MAXLEV];
+    hipStream_t stream = nullptr;
+    double *d_a[7] = {nullptr};         // operator, natural layout, FP64
+    double *d_f = nullptr, *d_b = nullptr, *d_res = nullptr;  // host-API staging
+    double *d_swap = nullptr;           // exchange buffer (FP64 elements)
+    long swap_cap = 0;
+    double *d_partial = nullptr;
+    long partial_cap = 0;
+    double *d_scalar = nullptr;         // 64 doubles of small device scratch
+    double *d_hist = nullptr;           // [cap][nlev][4]
+    long hist_cap = 0;
+    double *h_pinned = nullptr;         // pinned mirror of the history
+    long pinned_cap = 0;
+    int jumble = 0;
+    bool have_op = false;
+    // geometry (lmgx_build_operator)
+    double *d_met = nullptr;            // [7][n2d]: diu djv rdiv rdju didjp rdidjp rp
+    double *d_dkp = nullptr, *d_dkw = nullptr;
+    FastBufs *fast = nullptr;
+    // profiling of the smoother
+    bool prof = false;
+    double prof_ms = 0.0;
+    long prof_launch = 0;
+    double prof_cells = 0.0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
+    std::vector<double> prof_ev_cells;
+    size_t prof_used = 0;
+};
+
+static const int CYC[2][3] = {{2, 3, 1}, {3, 1, 2}};
+
+static bool is_swap(int bc) { return bc == BCT_INTERFACE || bc == BCT_PERIODIC; }
+
+// ---------------------------------------------------------------------------------------------
+// tables: level count and dims (reference code/set_offset.F:94-140), offsets (:166-234),
+// interior counts (code/set_loop_bound.F:87-104), face tables (code/set_interfaces.F:127-188)
+// ---------------------------------------------------------------------------------------------
+static int build_tables(lmgx_handle *h, const lmgx_topology *t)
+{
+    if (!t || t->nblk < 1 || t->nblk > LMGX_MAXB || !t->nijkp1 || !t->bc || !t->bl2bl ||
+        !t->bl2side)
+        return fail(LMGX_E_ARG, "bad topology");
+    h->nblk = t->nblk;
+    for (int b = 0; b < h->nblk; ++b) {
+        for (int d = 0; d < 3; ++d) {
+            h->nijkp1[0][b][d] = t->nijkp1[3 * b + d];
+            if (h->nijkp1[0][b][d] < 3) return fail(LMGX_E_ARG, "block dims must be >= 3");
+        }
+        for (int s = 0; s < 6; ++s) {
+            h->bc[b][s] = t->bc[6 * b + s];
+            h->bl2bl[b][s] = t->bl2bl[6 * b + s];
+            h->bl2side[b][s] = t->bl2side[6 * b + s];
+        }
+    }
+    int found = 1;
+    for (int l = 0; l < LMGX_MAXLEV - 1; ++l) {
+        bool ok = true;
+        for (int b = 0; b < h->nblk && ok; ++b) {
+            const int *n = h->nijkp1[l][b];
+            if (n[0] % 2 || n[1] % 2 || n[2] % 2) { ok = false; break; }
+            int c[3] = {n[0] / 2 + 1, n[1] / 2 + 1, n[2] / 2 + 1};
+            if (c[0] < 4 || c[1] < 4 || c[2] < 4) { ok = false; break; }
+            for (int d = 0; d < 3; ++d) h->nijkp1[l + 1][b][d] = c[d];
+        }
+        if (!ok) break;
+        found = l + 2;
+    }
+    int L = t->nlev == 0 ? found : t->nlev;
+    if (L < 1 || L > found)
+        return fail(LMGX_E_LEVELS, "requested " + std::to_string(t->nlev) + " levels, dims allow " +
+                                       std::to_string(found));
+    h->nlev = L;
+    long o = 0, o2 = 0, o1 = 0, oxy = 0, oz = 0;
+    for (int l = 0; l < L; ++l)
+        for (int b = 0; b < h->nblk; ++b) {
+            const int *n = h->nijkp1[l][b];
+            h->off[l][b] = o;
+            h->off2d[l][b] = o2;
+            h->off1d[l][b] = o1;
+            h->offxy[l][b] = oxy;
+            h->offz[l][b] = oz;
+            o += (long)n[0] * n[1] * n[2];
+            o2 += (long)n[0] * n[1];
+            o1 += n[2];
+            oxy += (long)(n[0] + 1) * (n[1] + 1);
+            oz += n[2] + 1;
+            if (l == 0) h->n1 = o;
+        }
+    h->ntot = o;
+    h->n2d = o2;
+    h->n1d = o1;
+    h->nxy = oxy;
+    h->nz = oz;
+    // face consistency (the reference prints a mismatch and carries on; here it is an error)
+    for (int b = 0; b < h->nblk; ++b)
+        for (int s = 1; s <= 6; ++s) {
+            int nb = h->bl2bl[b][s - 1], ns = h->bl2side[b][s - 1];
+            bool sw = is_swap(h->bc[b][s - 1]);
+            if (!sw) continue;
+            if (nb < 1 || nb > h->nblk || ns < 1 || ns > 6)
+                return fail(LMGX_E_FACE, "swapped side without a valid partner");
+            if (s % 2 == ns % 2) return fail(LMGX_E_FACE, "interface coupling invalid");
+            if (!is_swap(h->bc[nb - 1][ns - 1]))
+                return fail(LMGX_E_FACE, "partner side is not INTERFACE/PERIODIC");
+            for (int l = 0; l < L; ++l) {
+                int d1 = (s + 1) / 2, e1 = (ns + 1) / 2;
+                if (h->nijkp1[l][b][CYC[0][d1 - 1] - 1] != h->nijkp1[l][nb - 1][CYC[0][e1 - 1] - 1] ||
+                    h->nijkp1[l][b][CYC[1][d1 - 1] - 1] != h->nijkp1[l][nb - 1][CYC[1][e1 - 1] - 1])
+                    return fail(LMGX_E_FACE, "block " + std::to_string(b + 1) + " face " +
+                                                 std::to_string(s) + " mismatch");
+            }
+        }
+    for (int l = 0; l < L; ++l) {
+        LevelDev &D = h->lev[l];
+        D.h_lb.resize(h->nblk);
+        long ti = 0, ta = 0, tr = 0, sl = 0, ns = 0;
+        double np = 0.0;
+        long npi = 0;
+        int smax = 0;
+        for (int b = 0; b < h->nblk; ++b) {
+            const int *n = h->nijkp1[l][b];
+            LevBlk &B = D.h_lb[b];
+            memset(&B, 0, sizeof(B));
+            B.off = h->off[l][b];
+            B.n[0] = n[0]; B.n[1] = n[1]; B.n[2] = n[2];
+            B.hp = (n[0] + 1) / 2;
+            B.off_s = ns;
+            ns += (long)B.hp * n[1] * n[2];
+            for (int s = 0; s < 6; ++s) {
+                B.ifc[s] = h->bc[b][s] == BCT_INTERFACE;
+                B.swp[s] = is_swap(h->bc[b][s]);
+            }
+            B.t_int = ti;
+            ti += (long)((n[0] - 2 + 63) / 64) * ((n[1] - 2 + 3) / 4);
+            B.t_all = ta;
+            ta += (long)((n[0] + 63) / 64) * ((n[1] + 3) / 4);
+            B.t_rb = tr;
+            tr += (long)((B.hp + 63) / 64) * ((n[1] - 2 + 3) / 4);
+            B.s_jk = sl;
+            sl += (long)(n[1] - 2) * (n[2] - 2);
+            B.o2 = h->off2d[l][b];
+            B.o1 = h->off1d[l][b];
+            B.oxy = h->offxy[l][b];
+            B.oz = h->offz[l][b];
+            np = np + (double)((long)(n[0] - 2) * (n[1] - 2) * (n[2] - 2));
+            npi += (long)(n[0] - 2) * (n[1] - 2) * (n[2] - 2);
+            smax = std::max(smax, n[0] - 1 + n[1] - 1 + n[2] - 1);
+        }
+        D.tiles_int = ti;
+        D.tiles_all = ta;
+        D.tiles_rb = tr;
+        D.nlines = sl;
+        D.smax = smax;
+        D.npts_int = npi;
+        D.nsplit = ns;
+        h->npts[l] = np;
+        // faces, in block order and side order (the pack order of swap_sc0.F:64-151)
+        D.h_fc.clear();
+        long e = 0, buf = 0;
+        std::vector<long> slot(h->nblk * 6, -1);
+        for (int b = 0; b < h->nblk; ++b)
+            for (int s = 1; s <= 6; ++s) {
+                if (!is_swap(h->bc[b][s - 1])) continue;
+                const int *n = h->nijkp1[l][b];
+                int d1 = (s + 1) / 2, d2 = CYC[0][d1 - 1], d3 = CYC[1][d1 - 1];
+                long ijko[3] = {1, n[0], (long)n[0] * n[1]};
+                int ni = n[d1 - 1];
+                int ipk = (s % 2 == 0) ? ni - 1 : 2, iun = (s % 2 == 0) ? ni : 1;
+                Face F;
+                memset(&F, 0, sizeof(F));
+                F.pack_base = h->off[l][b] + ijko[d1 - 1] * (ipk - 1);
+                F.unpack_base = h->off[l][b] + ijko[d1 - 1] * (iun - 1);
+                F.jo = ijko[d2 - 1];
+                F.ko = ijko[d3 - 1];
+                F.nj = n[d2 - 1];
+                F.nk = n[d3 - 1];
+                F.buf = buf;
+                F.e0 = e;
+                F.blk = b;
+                F.ax_j = d2 - 1;
+                F.ax_k = d3 - 1;
+                for (int a = 0; a < 3; ++a) { F.pack_i[a] = 0; F.unpack_i[a] = 0; }
+                F.pack_i[d1 - 1] = ipk - 1;
+                F.unpack_i[d1 - 1] = iun - 1;
+                // points of this halo plane that a later swapped side of the block overwrites
+                int sk = 0;
+                if (2 * d2 - 1 > s && is_swap(h->bc[b][2 * d2 - 2])) sk |= 1;
+                if (2 * d2 > s && is_swap(h->bc[b][2 * d2 - 1])) sk |= 2;
+                if (2 * d3 - 1 > s && is_swap(h->bc[b][2 * d3 - 2])) sk |= 4;
+                if (2 * d3 > s && is_swap(h->bc[b][2 * d3 - 1])) sk |= 8;
+                F.skip = sk;
+                slot[b * 6 + s - 1] = buf;
+                buf += (long)F.nj * F.nk;
+                e += (long)F.nj * F.nk;
+                D.h_fc.push_back(F);
+            }
+        for (auto &F : D.h_fc) {
+            // which (block, side) is this?
+            int b = F.blk, s = 0;
+            for (int q = 1; q <= 6; ++q)
+                if (is_swap(h->bc[b][q - 1]) && slot[b * 6 + q - 1] == F.buf) s = q;
+            int nb = h->bl2bl[b][s - 1], nsd = h->bl2side[b][s - 1];
+            F.src = slot[(nb - 1) * 6 + nsd - 1];
+        }
+        D.nfaces = (int)D.h_fc.size();
+        D.nel = e;
+    }
+    return 0;
+}
+
+static int upload_tables(lmgx_handle *h)
+{
+    long swap_need = 1, part_need = 64;
+    for (int l = 0; l < h->nlev; ++l) {
+        LevelDev &D = h->lev[l];
+        HIPCHK(hipMalloc(&D.d_lb, sizeof(LevBlk) * h->nblk));
+        HIPCHK(hipMemcpy(D.d_lb, D.h_lb.data(), sizeof(LevBlk) * h->nblk, hipMemcpyHostToDevice));
+        if (D.nfaces) {
+            HIPCHK(hipMalloc(&D.d_fc, sizeof(Face) * D.nfaces));
+            HIPCHK(hipMemcpy(D.d_fc, D.h_fc.data(), sizeof(Face) * D.nfaces, hipMemcpyHostToDevice));
+        }
+        swap_need = std::max(swap_need, D.nel);
+        part_need = std::max(part_need, 4 * std::max(D.tiles_int, std::max(D.tiles_all, D.tiles_rb)));
+    }
+    HIPCHK(hipMalloc(&h->d_swap, sizeof(double) * swap_need));
+    h->swap_cap = swap_need;
+    HIPCHK(hipMalloc(&h->d_partial, sizeof(double) * part_need));
+    h->partial_cap = part_need;
+    HIPCHK(hipMalloc(&h->d_scalar, sizeof(double) * 64));
+    return 0;
+}
+
+extern "C" void lmgx_default_params(lmgx_params *p)
+{
+    // code/defaults.F:59-71
+    p->cycle = 1;
+    p->numpre = 1;
+    p->numpost = 1;
+    p->numcits = 20;
+    p->max_cycles = 5;
+    p->omega = 1.6;
+    p->omega_c = 1.6;
+    p->tolf = 5.0e-13;
+    p->rtol = 0.0;
+    p->smoother = LMGX_SMOOTH_EXACT;
+    p->precision = LMGX_PREC_FP64;
+    p->check_every = 1;
+    p->sweeps_per_iter = 3;
+}
+
+static void free_fast(lmgx_handle *h);
+
+extern "C" int lmgx_free(lmgx_handle *h)
+{
+    if (!h) return 0;
+    for (int l = 0; l < h->nlev; ++l) {
+        hipFree(h->lev[l].d_lb);
+        hipFree(h->lev[l].d_fc);
+    }
+    for (int q = 0; q < 7; ++q) hipFree(h->d_a[q]);
+    hipFree(h->d_f); hipFree(h->d_b); hipFree(h->d_res);
+    hipFree(h->d_swap); hipFree(h->d_partial); hipFree(h->d_scalar); hipFree(h->d_hist);
+    hipFree(h->d_met); hipFree(h->d_dkp); hipFree(h->d_dkw);
+    if (h->h_pinned) hipHostFree(h->h_pinned);
+    free_fast(h);
+    for (auto &e : h->prof_ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+extern "C" int lmgx_init(const lmgx_topology *topo, lmgx_handle **out)
+{
+    if (!out) return fail(LMGX_E_ARG, "out is NULL");
+    *out = nullptr;
+    lmgx_handle *h = new lmgx_handle();
+    int rc = build_tables(h, topo);
+    if (rc == 0) {
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+            rc = fail(LMGX_E_HIP, "hipStreamCreate failed");
+        } else {
+            rc = upload_tables(h);
+        }
+    }
+    if (rc != 0) {
+        std::string msg = g_err;
+        lmgx_free(h);
+        g_err = msg;
+        return rc;
+    }
+    *out = h;
+    return 0;
+}
+
+extern "C" int lmgx_levels(const lmgx_handle *h, int *nlev, long *ntot, long *n1,
+                           int *nijkp1_all, long *off_all)
+{
+    if (!h) return fail(LMGX_E_ARG, "NULL handle");
+    if (nlev) *nlev = h->nlev;
+    if (ntot) *ntot = h->ntot;
+    if (n1) *n1 = h->n1;
+    for (int l = 0; l < h->nlev; ++l)
+        for (int b = 0; b < h->nblk; ++b) {
+            if (nijkp1_all)
+                for (int d = 0; d < 3; ++d)
+                    nijkp1_all[(l * h->nblk + b) * 3 + d] = h->nijkp1[l][b][d];
+            if (off_all) off_all[l * h->nblk + b] = h->off[l][b];
+        }
+    return 0;
+}
+
+extern "C" int lmgx_get_jumble(const lmgx_handle *h) { return h ? h->jumble : 0; }
+extern "C" int lmgx_set_jumble(lmgx_handle *h, int j)
+{
+    if (!h) return fail(LMGX_E_ARG, "NULL handle");
+    h->jumble = j;
+    return 0;
+}
+
+static int ensure_operator_storage(lmgx_handle *h)
+{
+    for (int q = 0; q < 7; ++q)
+        if (!h->d_a[q]) HIPCHK(hipMalloc(&h->d_a[q], sizeof(double) * h->ntot));
+    return 0;
+}
+
+static int fast_operator_changed(lmgx_handle *h);
+
+extern "C" int lmgx_set_operator(lmgx_handle *h, const double *ap, const double *aw,
+                                 const double *ae, const double *as, const double *an,
+                                 const double *al, const double *ar)
+{
+    if (!h) return fail(LMGX_E_ARG, "NULL handle");
+    const double *src[7] = {ap, aw, ae, as, an, al, ar};
+    for (int q = 0; q < 7; ++q)
+        if (!src[q]) return fail(LMGX_E_ARG, "NULL coefficient array");
+    CHK(ensure_operator_storage(h));
+    for (int q = 0; q < 7; ++q)
+        HIPCHK(hipMemcpyAsync(h->d_a[q], src[q], sizeof(double) * h->ntot, hipMemcpyHostToDevice,
+                              h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->have_op = true;
+    return fast_operator_changed(h);
+}
+
+extern "C" int lmgx_set_operator_device(lmgx_handle *h, const double *const d_coeff[7])
+{
+    if (!h || !d_coeff) return fail(LMGX_E_ARG, "NULL argument");
+    CHK(ensure_operator_storage(h));
+    for (int q = 0; q < 7; ++q)
+        HIPCHK(hipMemcpyAsync(h->d_a[q], d_coeff[q], sizeof(double) * h->ntot,
+                              hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->have_op = true;
+    return fast_operator_changed(h);
+}
+
+extern "C" int lmgx_get_operator(const lmgx_handle *h, double *ap, double *aw, double *ae,
+                                 double *as, double *an, double *al, double *ar)
+{
+    if (!h || !h->have_op) return fail(LMGX_E_STATE, "no operator");
+    double *dst[7] = {ap, aw, ae, as, an, al, ar};
+    for (int q = 0; q < 7; ++q)
+        if (dst[q])
+            HIPCHK(hipMemcpyAsync(dst[q], h->d_a[q], sizeof(double) * h->ntot,
+                                  hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// reference-order passes
+// ---------------------------------------------------------------------------------------------
+static int ensure_hist(lmgx_handle *h, long n)
+{
+    if (n > h->hist_cap) {
+        hipFree(h->d_hist);
+        HIPCHK(hipMalloc(&h->d_hist, sizeof(double) * n));
+        h->hist_cap = n;
+    }
+    if (n > h->pinned_cap) {
+        if (h->h_pinned) hipHostFree(h->h_pinned);
+        HIPCHK(hipHostMalloc(&h->h_pinned, sizeof(double) * n, hipHostMallocDefault));
+        h->pinned_cap = n;
+    }
+    return 0;
+}
+
+// norms of level l into d_out[4] (device), residual into res
+static int norms_exact(lmgx_handle *h, int l, const double *x, const double *b, double *res,
+                       double *d_out)
+{
+    LevelDev &D = h->lev[l];
+    double *const *a = h->d_a;
+    hipLaunchKernelGGL(k_norms_exact, dim3(D.tiles_int), dim3(LMGX_WG), 0, h->stream, D.d_lb,
+                       h->nblk, a[0], a[1], a[2], a[3], a[4], a[5], a[6], x, b, res, h->d_partial);
+    hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), dim3(LMGX_WG), 0, h->stream, h->d_partial,
+                       D.tiles_int, 1.0 / h->npts[l], d_out, 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int swap_level(lmgx_handle *h, int l, T *x)
+{
+    LevelDev &D = h->lev[l];
+    if (D.nel == 0) return 0;
+    T *buf = reinterpret_cast<T *>(h->d_swap);
+    hipLaunchKernelGGL(k_swap_pack<T>, dim3(nblocks(D.nel)), dim3(LMGX_WG), 0, h->stream, D.d_fc,
+                       D.nfaces, D.nel, x, buf);
+    hipLaunchKernelGGL(k_swap_unpack<T>, dim3(nblocks(D.nel)), dim3(LMGX_WG), 0, h->stream, D.d_fc,
+                       D.nfaces, D.nel, buf, x);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static void prof_begin(lmgx_handle *h);
+static void prof_end(lmgx_handle *h, double cells);
+
+// one sporlx call: 3 sweeps, halo swap after each (code/sporlx.F:74-159)
+static int sporlx_exact(lmgx_handle *h, int l, double *f, const double *b, double omega)
+{
+    int jumfor = (h->jumble % 16 >= 8) ? 1 : 2;
+    int jumst = (h->jumble % 24) / 8 + 1;
+    int d[3] = {jumst, CYC[(jumfor + 1) % 2][jumst - 1], CYC[jumfor % 2][jumst - 1]};
+    LevelDev &D = h->lev[l];
+    double *const *a = h->d_a;
+    for (int il = 0; il < 3; ++il) {
+        prof_begin(h);
+        for (int s = 6; s <= D.smax; ++s)
+            hipLaunchKernelGGL(k_sor_hyperplane, dim3(nblocks(D.nlines)), dim3(LMGX_WG), 0,
+                               h->stream, D.d_lb, h->nblk, D.nlines, s, d[il], omega, a[0], a[1],
+                               a[2], a[3], a[4], a[5], a[6], f, b);
+        prof_end(h, (double)D.npts_int);
+        CHK(swap_level<double>(h, l, f));
+    }
+    HIPCHK(hipGetLastError());
+    h->jumble = h->jumble + 13;
+    if (h->jumble > 99999) h->jumble = h->jumble % 99999;
+    return 0;
+}
+
+static int restrict_exact(lmgx_handle *h, int lc, const double *res, double *b, double *f)
+{
+    LevelDev &C = h->lev[lc];
+    hipLaunchKernelGGL(k_restrict_zero<double>, dim3(C.tiles_all), dim3(LMGX_WG), 0, h->stream,
+                       C.d_lb, h->lev[lc - 1].d_lb, h->nblk, res, b, f);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int prolong_exact(lmgx_handle *h, int lf, double *f)
+{
+    LevelDev &F = h->lev[lf], &C = h->lev[lf + 1];
+    hipLaunchKernelGGL(k_zero_grad<double>, dim3(C.tiles_all), dim3(LMGX_WG), 0, h->stream, C.d_lb,
+                       h->nblk, f, 0);
+    hipLaunchKernelGGL(k_prolong_exact<double>, dim3(F.tiles_all), dim3(LMGX_WG), 0, h->stream,
+                       F.d_lb, C.d_lb, h->nblk, (const double *)f, f);
+    HIPCHK(hipGetLastError());
+    return swap_level<double>(h, lf, f);
+}
+
+// code/smooth.F:34-100 (the smoothing stop test never fires with soltol = 0 unless a sweep
+// leaves f exactly unchanged; this path always runs the full count)
+static int smooth_exact(lmgx_handle *h, int l, double *f, double *b, double *res, int nsmooth,
+                        double omega, double *d_nrm)
+{
+    for (int it = 0; it < nsmooth; ++it) CHK(sporlx_exact(h, l, f, b, omega));
+    return norms_exact(h, l, f, b, res, d_nrm);
+}
+
+static int read_norms(lmgx_handle *h, const double *d_src, double *host4)
+{
+    HIPCHK(hipMemcpyAsync(h->h_pinned, d_src, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    memcpy(host4, h->h_pinned, 4 * sizeof(double));
+    return 0;
+}
+
+// code/lmg_solve.F:44-160 with nummgits = 1 per call, repeated up to max_cycles times
+static int solve_exact(lmgx_handle *h, double *f, double *b, double *res, const lmgx_params *p,
+                       double *history, int *ncyc)
+{
+    const int L = h->nlev;
+    // device norm slots: current[l][4] in d_scalar (l < 8 -> 32 doubles)
+    double *d_nrm = h->d_scalar;
+    double n0[4];
+    double rnorm0 = -1.0;
+    int done = 0;
+    HIPCHK(hipMemsetAsync(d_nrm, 0, sizeof(double) * 4 * LMGX_MAXLEV, h->stream));
+    CHK(norms_exact(h, 0, f, b, res, d_nrm));
+    CHK(read_norms(h, d_nrm, n0));
+    rnorm0 = n0[0];
+    double cur[4] = {n0[0], n0[1], n0[2], n0[3]};
+    for (int call = 0; call < p->max_cycles; ++call) {
+        if (!(cur[2] / cur[3] > p->tolf)) break;
+        if (p->rtol > 0.0 && rnorm0 > 0.0 && cur[0] / rnorm0 <= p->rtol) break;
+        const bool fcycle = p->cycle == 2;
+        int gamma = fcycle ? 2 : 1;
+        int nummg[LMGX_MAXLEV];
+        nummg[0] = 1;
+        int il = 0;
+        // the per-call entry norms (lmg_solve.F:60) equal the previous exit norms bit for bit
+        while (nummg[0] > 0) {
+            bool goto_fine = (il == L - 1) || (nummg[il] == 0);
+            if (goto_fine) {
+                if (il == L - 1) {
+                    CHK(norms_exact(h, il, f, b, res, d_nrm + 4 * il));
+                    CHK(smooth_exact(h, il, f, b, res, p->numcits, p->omega_c, d_nrm + 4 * il));
+                    if (fcycle) gamma = 1;
+                }
+                if (il > 0 && L != 1) {
+                    il = il - 1;
+                    CHK(prolong_exact(h, il, f));
+                    if (il == 0) CHK(norms_exact(h, il, f, b, res, d_nrm));
+                    CHK(smooth_exact(h, il, f, b, res, p->numpost, p->omega, d_nrm + 4 * il));
+                    if (il == 0 && fcycle) gamma = 2;
+                }
+                nummg[il] = nummg[il] - 1;
+            } else {
+                if (il == 0) CHK(norms_exact(h, il, f, b, res, d_nrm));
+                CHK(smooth_exact(h, il, f, b, res, p->numpre, p->omega, d_nrm + 4 * il));
+                il = il + 1;
+                nummg[il] = gamma;
+                CHK(restrict_exact(h, il, res, b, f));
+            }
+        }
+        HIPCHK(hipMemcpyAsync(h->d_hist + (long)call * L * 4, d_nrm, sizeof(double) * 4 * L,
+                              hipMemcpyDeviceToDevice, h->stream));
+        CHK(read_norms(h, d_nrm, cur));
+        ++done;
+        if (!std::isfinite(cur[0])) {
+            *ncyc = done;
+            return fail(LMGX_E_NUMERIC, "non-finite residual norm");
+        }
+    }
+    if (history && done) {
+        HIPCHK(hipMemcpyAsync(h->h_pinned, h->d_hist, sizeof(double) * 4 * L * done,
+                              hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        memcpy(history, h->h_pinned, sizeof(double) * 4 * L * done);
+    }
+    *ncyc = done;
+    return 0;
+}
+
+static int solve_fast(lmgx_handle *h, double *f, double *b, double *res, const lmgx_params *p,
+                      double *history, int *ncyc);
+
+static int check_params(const lmgx_handle *h, const lmgx_params *p)
+{
+    if (!p) return fail(LMGX_E_ARG, "NULL params");
+    if (p->cycle != 1 && p->cycle != 2) return fail(LMGX_E_ARG, "cycle must be 1 (V) or 2 (F)");
+    if (p->numpre < 0 || p->numpost < 0 || p->numcits < 0 || p->max_cycles < 0)
+        return fail(LMGX_E_ARG, "negative iteration count");
+    if (p->smoother != LMGX_SMOOTH_EXACT && p->smoother != LMGX_SMOOTH_RB)
+        return fail(LMGX_E_ARG, "unknown smoother");
+    if (p->precision != LMGX_PREC_FP64 && p->precision != LMGX_PREC_MIXED)
+        return fail(LMGX_E_ARG, "unknown precision");
+    if (p->precision == LMGX_PREC_MIXED && p->smoother != LMGX_SMOOTH_RB)
+        return fail(LMGX_E_ARG, "mixed precision needs the red-black smoother");
+    if (p->smoother == LMGX_SMOOTH_RB && p->cycle != 1)
+        return fail(LMGX_E_UNSUPPORTED, "red-black path runs V-cycles only");
+    if (!h->have_op) return fail(LMGX_E_STATE, "lmgx_set_operator has not been called");
+    return 0;
+}
+
+extern "C" int lmgx_solve_device(lmgx_handle *h, double *d_f, double *d_b, double *d_res,
+                                 const lmgx_params *prm, double *history, int *ncycles_out)
+{
+    if (!h || !d_f || !d_b || !d_res) return fail(LMGX_E_ARG, "NULL argument");
+    CHK(check_params(h, prm));
+    int nc = 0;
+    CHK(ensure_hist(h, (long)std::max(1, prm->max_cycles) * h->nlev * 4 + 64));
+    int rc = prm->smoother == LMGX_SMOOTH_EXACT ? solve_exact(h, d_f, d_b, d_res, prm, history, &nc)
+                                                : solve_fast(h, d_f, d_b, d_res, prm, history, &nc);
+    if (ncycles_out) *ncycles_out = nc;
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int lmgx_solve(lmgx_handle *h, double *f, double *b, double *residu,
+                          const lmgx_params *prm, double *history, int *ncycles_out)
+{
+    if (!h || !f || !b || !residu) return fail(LMGX_E_ARG, "NULL argument");
+    CHK(check_params(h, prm));
+    const size_t bytes = sizeof(double) * h->ntot;
+    if (!h->d_f) {
+        HIPCHK(hipMalloc(&h->d_f, bytes));
+        HIPCHK(hipMalloc(&h->d_b, bytes));
+        HIPCHK(hipMalloc(&h->d_res, bytes));
+    }
+    HIPCHK(hipMemcpyAsync(h->d_f, f, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_b, b, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_res, residu, bytes, hipMemcpyHostToDevice, h->stream));
+    int rc = lmgx_solve_device(h, h->d_f, h->d_b, h->d_res, prm, history, ncycles_out);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(f, h->d_f, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(b, h->d_b, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(residu, h->d_res, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// single passes
+// ---------------------------------------------------------------------------------------------
+static int check_level(const lmgx_handle *h, int l)
+{
+    if (!h) return fail(LMGX_E_ARG, "NULL handle");
+    if (l < 0 || l >= h->nlev) return fail(LMGX_E_ARG, "level out of range");
+    return 0;
+}
+
+extern "C" int lmgx_norms_device(lmgx_handle *h, int level, const double *d_x, const double *d_b,
+                                 double *d_res, double norms_out[4])
+{
+    CHK(check_level(h, level));
+    if (!h->have_op) return fail(LMGX_E_STATE, "no operator");
+    CHK(ensure_hist(h, 64));
+    CHK(norms_exact(h, level, d_x, d_b, d_res, h->d_scalar));
+    double tmp[4];
+    CHK(read_norms(h, h->d_scalar, tmp));
+    if (norms_out) memcpy(norms_out, tmp, sizeof(tmp));
+    return 0;
+}
+
+extern "C" int lmgx_sporlx_device(lmgx_handle *h, int level, double *d_f, const double *d_b,
+                                  double omega)
+{
+    CHK(check_level(h, level));
+    if (!h->have_op) return fail(LMGX_E_STATE, "no operator");
+    CHK(sporlx_exact(h, level, d_f, d_b, omega));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int lmgx_swap_device(lmgx_handle *h, int level, double *d_var)
+{
+    CHK(check_level(h, level));
+    CHK(swap_level<double>(h, level, d_var));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int lmgx_restrict_device(lmgx_handle *h, int lc, const double *d_res, double *d_b,
+                                    double *d_f)
+{
+    CHK(check_level(h, lc));
+    if (lc == 0) return fail(LMGX_E_ARG, "coarse level must be >= 1");
+    CHK(restrict_exact(h, lc, d_res, d_b, d_f));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int lmgx_prolong_device(lmgx_handle *h, int lf, double *d_f)
+{
+    CHK(check_level(h, lf));
+    if (lf >= h->nlev - 1) return fail(LMGX_E_ARG, "fine level must be < nlev-1");
+    CHK(prolong_exact(h, lf, d_f));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int lmgx_extrap_p_device(lmgx_handle *h, double *d_p)
+{
+    if (!h || !d_p) return fail(LMGX_E_ARG, "NULL argument");
+    LevelDev &D = h->lev[0];
+    hipLaunchKernelGGL(k_zero_grad<double>, dim3(D.tiles_all), dim3(LMGX_WG), 0, h->stream, D.d_lb,
+                       h->nblk, d_p, 1);
+    HIPCHK(hipGetLastError());
+    CHK(swap_level<double>(h, 0, d_p));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int lmgx_p_ref_subtract_device(lmgx_handle *h, double *d_p)
+{
+    if (!h || !d_p) return fail(LMGX_E_ARG, "NULL argument");
+    const int *n = h->nijkp1[0][0];
+    long idx = h->off[0][0] + 1 + (long)n[0] + (long)n[0] * n[1];
+    hipLaunchKernelGGL(k_pref_read, dim3(1), dim3(LMGX_WG), 0, h->stream, d_p, idx,
+                       h->d_scalar + 40);
+    hipLaunchKernelGGL(k_sub_scalar, dim3(nblocks(h->n1)), dim3(LMGX_WG), 0, h->stream, d_p, h->n1,
+                       (const double *)(h->d_scalar + 40));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int lmgx_global_sum_device(lmgx_handle *h, int level, const double *d_f,
+                                      double out[3])
+{
+    CHK(check_level(h, level));
+    LevelDev &D = h->lev[level];
+    hipLaunchKernelGGL(k_gsum, dim3(D.tiles_int), dim3(LMGX_WG), 0, h->stream, D.d_lb, h->nblk, d_f,
+                       h->d_partial);
+    hipLaunchKernelGGL(k_gsum_final, dim3(1), dim3(LMGX_WG), 0, h->stream,
+                       (const double *)h->d_partial, D.tiles_int, h->d_scalar + 44);
+    HIPCHK(hipGetLastError());
+    CHK(ensure_hist(h, 64));
+    HIPCHK(hipMemcpyAsync(h->h_pinned, h->d_scalar + 44, 3 * sizeof(double),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    memcpy(out, h->h_pinned, 3 * sizeof(double));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// smoother profiling (HIP events on the handle's stream around every smoother launch group)
+// ---------------------------------------------------------------------------------------------
+extern "C" int lmgx_profile(lmgx_handle *h, int enable)
+{
+    if (!h) return fail(LMGX_E_ARG, "NULL handle");
+    h->prof = enable != 0;
+    h->prof_ms = 0.0;
+    h->prof_launch = 0;
+    h->prof_cells = 0.0;
+    h->prof_used = 0;
+    h->prof_ev_cells.clear();
+    return 0;
+}
+
+static void prof_begin(lmgx_handle *h)
+{
+    if (!h->prof) return;
+    if (h->prof_used == h->prof_ev.size()) {
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        h->prof_ev.push_back({a, b});
+    }
+    hipEventRecord(h->prof_ev[h->prof_used].first, h->stream);
+}
+
+static void prof_end(lmgx_handle *h, double cells)
+{
+    if (!h->prof) return;
+    hipEventRecord(h->prof_ev[h->prof_used].second, h->stream);
+    h->prof_ev_cells.push_back(cells);
+    ++h->prof_used;
+}
+
+static void prof_collect(lmgx_handle *h)
+{
+    if (!h->prof || h->prof_used == 0) return;
+    hipStreamSynchronize(h->stream);
+    for (size_t q = 0; q < h->prof_used; ++q) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, h->prof_ev[q].first, h->prof_ev[q].second);
+        h->prof_ms += ms;
+        h->prof_cells += h->prof_ev_cells[q];
+        ++h->prof_launch;
+    }
+    h->prof_used = 0;
+    h->prof_ev_cells.clear();
+}
+
+extern "C" int lmgx_profile_read(const lmgx_handle *hc, double *ms, long *launches, double *cells)
+{
+    lmgx_handle *h = const_cast<lmgx_handle *>(hc);
+    if (!h) return fail(LMGX_E_ARG, "NULL handle");
+    prof_collect(h);
+    if (ms) *ms = h->prof_ms;
+    if (launches) *launches = h->prof_launch;
+    if (cells) *cells = h->prof_cells;
+    return 0;
+}
+
+#include "fast_path.inc"
+#include "operator_build.inc"
+#include "adjoint.inc"

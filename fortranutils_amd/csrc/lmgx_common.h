@@ -1,0 +1,68 @@
+// Shared device/host structures of liblmgx (MI355X / gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LMGX_MAXB 64
+#define LMGX_MAXLEV 8
+#define LMGX_WG 256
+
+#define BCT_PERIODIC 12
+#define BCT_INTERFACE 13
+
+// One block on one level, natural (reference) layout.
+struct LevBlk {
+    long off;        // mg_off: start of the block in the multi-level array
+    long off_s;      // start of the block in each colour array of the split layout
+    int n[3];        // nip1, njp1, nkp1 (halos included)
+    int hp;          // slots per (j,k) line per colour in the split layout = ceil(nip1/2)
+    int ifc[6];      // side is an INTERFACE (correction range covers the halo)
+    int swp[6];      // side is INTERFACE or PERIODIC (halo swapped)
+    long t_int;      // first tile of this block in the interior (i,j) tiling
+    long t_all;      // first tile of this block in the full (i,j) tiling (halos included)
+    long t_rb;       // first tile of this block in the split-layout (slot,j) tiling
+    long s_jk;       // first (j,k) interior line of this block (hyperplane sweep)
+    long o2;         // offset of the block's 2-D (i,j) metric arrays
+    long o1;         // offset of the block's 1-D k metric arrays
+    long oxy;        // offset of the block's vertex array (nip1+1)*(njp1+1)
+    long oz;         // offset of the block's k-vertex array (nkp1+1)
+};
+
+// One swapped face of one block on one level (reference swap_sc0 tables).
+struct Face {
+    long pack_base;    // first element of the packed plane (plane 2 or n-1), natural layout
+    long unpack_base;  // first element of the halo plane (plane 1 or n)
+    long jo, ko;       // strides along the two in-plane axes (cyclic order)
+    int nj, nk;        // face extents (halos included)
+    long buf;          // start of this face's packed data in the exchange buffer
+    long src;          // start of the partner face's packed data (what lands in our halo)
+    long e0;           // first element index of this face in the flattened element list
+    int skip;          // bit0: j==1 overwritten later, bit1: j==nj, bit2: k==1, bit3: k==nk
+    int blk;           // block (for the split layout)
+    int pack_i[3];     // natural (i,j,k)-0 based of the pack plane origin
+    int unpack_i[3];   // natural origin of the halo plane
+    int ax_j, ax_k;    // 0-based axes of the in-plane directions
+};
+
+__host__ __device__ inline int xcd_remap(int b, int G)
+{
+    // Workgroups are dealt round-robin over the 8 XCDs; give each XCD a contiguous tile range so
+    // that neighbouring tiles (which share halo lines) meet in the same L2.
+    const int nx = 8;
+    int q = G / nx, r = G % nx;
+    int x = b % nx, i = b / nx;
+    return x * q + (x < r ? x : r) + i;
+}
+
+// Find the block owning flattened tile t (start[] ascending, nblk <= 64).
+__device__ inline int find_block(const LevBlk *lb, int nblk, long t, int which)
+{
+    int lo = 0, hi = nblk - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        long s = which == 0 ? lb[mid].t_int : which == 1 ? lb[mid].t_all : which == 2 ? lb[mid].t_rb
+                                                                                       : lb[mid].s_jk;
+        if (s <= t) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}

@@ -1,0 +1,241 @@
+"""Python binding of liblmgx (ctypes over the C ABI in ``include/lmgx.h``).
+
+Mirrors the reference's pressure-solve interface, ``lmg_solve(ap,aw,ae,as,an,al,ar,f,b,residu,
+ia,ra)`` (reference ``code/lmg_solve.F:5``): coefficient arrays, solution ``f``, source ``b`` and
+residual ``residu`` in the reference's multi-level layout.  The reference takes its topology and
+multigrid knobs from common blocks (``/cblgrid1_1/``, ``/cbmgdata1/``, ``/cbmgdata2/``); here they
+are the :class:`Topology` given to :class:`Solver` and the keyword arguments of
+:meth:`Solver.lmg_solve`.  The reference stops the program on a bad configuration
+(``code/solvers.F:35-36``, ``code/smooth.F:77-79``); this binding raises :class:`LmgxError`.
+
+The HIP library is required: nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblmgx.so")
+
+LMGX_SMOOTH_EXACT, LMGX_SMOOTH_RB = 0, 1
+LMGX_PREC_FP64, LMGX_PREC_MIXED = 0, 1
+MAXL = 8
+
+
+class LmgxError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("cycle", C.c_int), ("numpre", C.c_int), ("numpost", C.c_int),
+                ("numcits", C.c_int), ("max_cycles", C.c_int), ("omega", C.c_double),
+                ("omega_c", C.c_double), ("tolf", C.c_double), ("rtol", C.c_double),
+                ("smoother", C.c_int), ("precision", C.c_int), ("check_every", C.c_int),
+                ("sweeps_per_iter", C.c_int)]
+
+
+class _Topo(C.Structure):
+    _fields_ = [("nblk", C.c_int), ("nlev", C.c_int), ("nijkp1", C.POINTER(C.c_int)),
+                ("bc", C.POINTER(C.c_int)), ("bl2bl", C.POINTER(C.c_int)),
+                ("bl2side", C.POINTER(C.c_int))]
+
+
+_lib = None
+
+
+def lib():
+    """Load liblmgx.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LmgxError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        L.lmgx_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise LmgxError(f"lmgx error {rc}: {lib().lmgx_last_error().decode()}")
+
+
+def _dp(a) -> C.POINTER(C.c_double):
+    """Pointer to a float64 numpy array or a torch tensor (host or device)."""
+    if isinstance(a, np.ndarray):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+    # torch tensor
+    assert a.is_contiguous(), "tensor must be contiguous"
+    return C.cast(C.c_void_p(a.data_ptr()), C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+@dataclass
+class Topology:
+    """Block dims (halos included), side codes and links, reference ``code/cb.lgrid.inc``."""
+    nijkp1: np.ndarray     # [nblk, 3]
+    bc: np.ndarray         # [nblk, 6]
+    bl2bl: np.ndarray      # [nblk, 6]
+    bl2side: np.ndarray    # [nblk, 6]
+    nlev: int = 0
+
+    @classmethod
+    def from_problem(cls, p) -> "Topology":
+        return cls(np.array([b.nijkp1 for b in p.blocks], np.int32),
+                   np.array([b.bc for b in p.blocks], np.int32),
+                   np.array([b.bl2bl for b in p.blocks], np.int32),
+                   np.array([b.bl2side for b in p.blocks], np.int32), p.nummgl)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().lmgx_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Solver:
+    """One LESter block topology resident on the GPU (the reference's commons + arena)."""
+
+    def __init__(self, topo: Topology):
+        L = lib()
+        self._keep = [np.ascontiguousarray(a, np.int32).ravel()
+                      for a in (topo.nijkp1, topo.bc, topo.bl2bl, topo.bl2side)]
+        t = _Topo(len(topo.nijkp1), topo.nlev, *[_ip(a) for a in self._keep])
+        h = C.c_void_p()
+        _check(L.lmgx_init(C.byref(t), C.byref(h)))
+        self.h = h
+        nlev, ntot, n1 = C.c_int(), C.c_long(), C.c_long()
+        nb = len(topo.nijkp1)
+        dims = np.zeros((MAXL, nb, 3), np.int32)
+        off = np.zeros((MAXL, nb), np.int64)
+        _check(L.lmgx_levels(h, C.byref(nlev), C.byref(ntot), C.byref(n1), _ip(dims),
+                             off.ctypes.data_as(C.POINTER(C.c_long))))
+        self.nlev, self.ntot, self.n1, self.nblk = nlev.value, ntot.value, n1.value, nb
+        self.nijkp1 = dims[: self.nlev].copy()
+        self.off = off[: self.nlev].copy()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lmgx_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- operator -----------------------------------------------------------------------
+    def set_operator(self, coeffs: Sequence):
+        """ap, aw, ae, as, an, al, ar (ntot each): numpy (host) or torch CUDA tensors."""
+        if isinstance(coeffs[0], np.ndarray):
+            _check(lib().lmgx_set_operator(self.h, *[_dp(np.ascontiguousarray(a, np.float64))
+                                                     for a in coeffs]))
+        else:
+            arr = (C.POINTER(C.c_double) * 7)(*[_dp(a) for a in coeffs])
+            _check(lib().lmgx_set_operator_device(self.h, arr))
+
+    def build_operator(self, problem, rho1: Optional[np.ndarray] = None, dirichlet=None):
+        """Metrics + coefficients of every level on the device from the level-1 vertices."""
+        xv = np.concatenate([np.broadcast_to(b.xv[None, :], (b.nijkp1[1] - 1, b.nijkp1[0] - 1)).ravel()
+                             for b in problem.blocks]).astype(np.float64)
+        yv = np.concatenate([np.broadcast_to(b.yv[:, None], (b.nijkp1[1] - 1, b.nijkp1[0] - 1)).ravel()
+                             for b in problem.blocks]).astype(np.float64)
+        zv = np.concatenate([b.zv for b in problem.blocks]).astype(np.float64)
+        r = None if rho1 is None else np.ascontiguousarray(rho1, np.float64)
+        d = None if dirichlet is None else np.ascontiguousarray(dirichlet, np.int32).ravel()
+        _check(lib().lmgx_build_operator(self.h, _dp(xv), _dp(yv), _dp(zv),
+                                         None if r is None else _dp(r),
+                                         None if d is None else _ip(d)))
+
+    def get_operator(self):
+        a = [np.zeros(self.ntot) for _ in range(7)]
+        _check(lib().lmgx_get_operator(self.h, *[_dp(x) for x in a]))
+        return a
+
+    def source(self, u, v, w, rho, rhonp1, dtime, b):
+        _check(lib().lmgx_source_device(self.h, _dp(u), _dp(v), _dp(w), _dp(rho), _dp(rhonp1),
+                                        C.c_double(dtime), _dp(b)))
+
+    # ---- solve --------------------------------------------------------------------------
+    def lmg_solve(self, f, b, residu, *, max_cycles=5, history=True, **kw):
+        """Run up to ``max_cycles`` V-cycles.  numpy arrays -> host entry point, torch CUDA
+        tensors -> device entry point.  Returns the history [ncycles, nlev, 4] of
+        (rnorm, fnorm, enorm, vnorm)."""
+        p = default_params(max_cycles=max_cycles, **kw)
+        hist = np.zeros((max(1, max_cycles), self.nlev, 4))
+        nc = C.c_int()
+        if isinstance(f, np.ndarray):
+            _check(lib().lmgx_solve(self.h, _dp(f), _dp(b), _dp(residu), C.byref(p), _dp(hist),
+                                    C.byref(nc)))
+        else:
+            _check(lib().lmgx_solve_device(self.h, _dp(f), _dp(b), _dp(residu), C.byref(p),
+                                           _dp(hist), C.byref(nc)))
+        return hist[: nc.value]
+
+    def solve_adjoint(self, lam, g, res, *, max_cycles=50, **kw):
+        p = default_params(max_cycles=max_cycles, **kw)
+        hist = np.zeros((max(1, max_cycles), self.nlev, 4))
+        nc = C.c_int()
+        _check(lib().lmgx_solve_adjoint_device(self.h, _dp(lam), _dp(g), _dp(res), C.byref(p),
+                                               _dp(hist), C.byref(nc)))
+        return hist[: nc.value]
+
+    @property
+    def jumble(self) -> int:
+        return lib().lmgx_get_jumble(self.h)
+
+    @jumble.setter
+    def jumble(self, v: int):
+        _check(lib().lmgx_set_jumble(self.h, int(v)))
+
+    # ---- single passes (device tensors) -------------------------------------------------
+    def norms(self, level, x, b, res):
+        out = np.zeros(4)
+        _check(lib().lmgx_norms_device(self.h, level, _dp(x), _dp(b), _dp(res), _dp(out)))
+        return out
+
+    def sporlx(self, level, f, b, omega):
+        _check(lib().lmgx_sporlx_device(self.h, level, _dp(f), _dp(b), C.c_double(omega)))
+
+    def swap(self, level, var):
+        _check(lib().lmgx_swap_device(self.h, level, _dp(var)))
+
+    def restrict(self, coarse_level, res, b, f):
+        _check(lib().lmgx_restrict_device(self.h, coarse_level, _dp(res), _dp(b), _dp(f)))
+
+    def prolong(self, fine_level, f):
+        _check(lib().lmgx_prolong_device(self.h, fine_level, _dp(f)))
+
+    def extrap_p(self, p):
+        _check(lib().lmgx_extrap_p_device(self.h, _dp(p)))
+
+    def p_ref_subtract(self, p):
+        _check(lib().lmgx_p_ref_subtract_device(self.h, _dp(p)))
+
+    def global_sum(self, level, f):
+        out = np.zeros(3)
+        _check(lib().lmgx_global_sum_device(self.h, level, _dp(f), _dp(out)))
+        return out
+
+    # ---- profiling ----------------------------------------------------------------------
+    def profile(self, enable=True):
+        _check(lib().lmgx_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        ms, n, cells = C.c_double(), C.c_long(), C.c_double()
+        _check(lib().lmgx_profile_read(self.h, C.byref(ms), C.byref(n), C.byref(cells)))
+        return ms.value, n.value, cells.value

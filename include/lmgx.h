@@ -1,0 +1,162 @@
+/* lmgx - MI355X-native pressure-Poisson multigrid for LESter's multiblock layout.
+ *
+ * C ABI of liblmgx.so.  Plain pointers and sizes only.  Every entry point returns 0 on success
+ * or a negative LMGX_E* code; lmgx_last_error() gives a message for the calling thread.
+ *
+ * Data layout (same as the reference): multi-level arrays concatenate the blocks of level 1,
+ * then the blocks of level 2, ... (reference code/set_offset.F:166-234); inside a block the
+ * index is (i-1) + nip1*(j-1) + nip1*njp1*(k-1), i fastest, one halo layer per side
+ * (code/norms_block.F:36-45).  Level and block indices in this API are 0-based.
+ *
+ * Threading: one host thread per handle at a time.  A handle owns its device memory and one
+ * HIP stream; host pointers are borrowed for the duration of a call only.
+ */
+#ifndef LMGX_H
+#define LMGX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMGX_OK 0
+#define LMGX_E_ARG (-1)      /* bad argument / topology */
+#define LMGX_E_LEVELS (-2)   /* more levels requested than the block dims allow */
+#define LMGX_E_FACE (-3)     /* interface faces do not match */
+#define LMGX_E_HIP (-4)      /* HIP runtime error */
+#define LMGX_E_STATE (-5)    /* call out of order (e.g. solve before set_operator) */
+#define LMGX_E_NUMERIC (-6)  /* NaN/Inf in the residual norms */
+#define LMGX_E_UNSUPPORTED (-7)
+
+/* boundary codes, reference code/bc_type.inc:17-27 (only these two change the solver) */
+#define LMGX_BC_PERIODIC 12
+#define LMGX_BC_INTERFACE 13
+
+typedef struct lmgx_handle lmgx_handle;
+
+typedef struct lmgx_topology {
+    int nblk;              /* blocks (reference numbl) */
+    int nlev;              /* levels wanted, 0 = as many as the dims allow (nummgl = 0) */
+    const int *nijkp1;     /* [nblk][3] level-1 dims incl. halos (bl_nijkp1) */
+    const int *bc;         /* [nblk][6] side codes (bl_bc) */
+    const int *bl2bl;      /* [nblk][6] partner block, 1-based, 0 = none (bl2bl) */
+    const int *bl2side;    /* [nblk][6] partner side 1..6 (bl2side) */
+} lmgx_topology;
+
+/* smoother */
+#define LMGX_SMOOTH_EXACT 0   /* reference order: 3 lexicographic SOR sweeps per iteration */
+#define LMGX_SMOOTH_RB 1      /* red-black SOR, colour-split layout */
+/* precision */
+#define LMGX_PREC_FP64 0
+#define LMGX_PREC_MIXED 1     /* FP32 V-cycle inside FP64 iterative refinement */
+
+typedef struct lmgx_params {
+    int cycle;            /* 1 = V-cycle (meth_cycle), 2 = F-cycle (exact smoother only) */
+    int numpre, numpost;  /* smoothing iterations per level (numpre/numpost) */
+    int numcits;          /* smoothing iterations on the coarsest level (numcits) */
+    int max_cycles;       /* upper bound on V-cycles (nummgits) */
+    double omega;         /* relax_pmg */
+    double omega_c;       /* relax_pmgc */
+    double tolf;          /* reference stop test enorm/vnorm <= tolf (lmg_solve.F:82) */
+    double rtol;          /* extra stop test rnorm_k/rnorm_0 <= rtol, 0 = off */
+    int smoother;         /* LMGX_SMOOTH_* */
+    int precision;        /* LMGX_PREC_* (MIXED needs LMGX_SMOOTH_RB) */
+    int check_every;      /* fast path: test convergence every N cycles (>= 1) */
+    int sweeps_per_iter;  /* RB smoother: RB sweeps per smoothing iteration (reference: 3) */
+} lmgx_params;
+
+/* Default parameters: the reference's hard-wired knobs (code/defaults.F:59-71). */
+void lmgx_default_params(lmgx_params *p);
+
+const char *lmgx_last_error(void);
+
+/* Replaces the table set-up of code/set_offset.F:8-263, code/set_loop_bound.F:11-125 and
+ * code/set_interfaces.F:6-256 (level count, per-level per-block offsets, interior ranges, face
+ * copy tables) and allocates the device mirrors. */
+int lmgx_init(const lmgx_topology *topo, lmgx_handle **out);
+int lmgx_free(lmgx_handle *h);
+
+/* Level tables: nlev, total points over all levels, level-1 points, and per level/block the
+ * dims and offsets (arrays sized [LMGX_MAXL][nblk]...; any pointer may be NULL). */
+#define LMGX_MAXL 8
+int lmgx_levels(const lmgx_handle *h, int *nlev, long *ntot, long *n1, int *nijkp1_all,
+                long *off_all);
+
+/* Stencil coefficients ap..ar for every level (host arrays, ntot doubles each), as produced by
+ * code/coeff_p.F:5-78 and consumed by code/lmg_solve.F:5 (its first seven arguments). */
+int lmgx_set_operator(lmgx_handle *h, const double *ap, const double *aw, const double *ae,
+                      const double *as, const double *an, const double *al, const double *ar);
+/* Same, from device pointers (e.g. torch tensors). */
+int lmgx_set_operator_device(lmgx_handle *h, const double *const d_coeff[7]);
+
+/* The solve.  Replaces code/lmg_solve.F:5-172 with everything it calls: smooth.F, sporlx.F,
+ * solversc.F, norms.F/norms_block.F, restriction.F/rest_block.F, prolongation.F/prol_block.F,
+ * swap_sc0.F, init_array.F.  f: initial guess in (level 1), solution out (all levels); b: source
+ * in (level 1), coarse right-hand sides out; residu: residuals out (all levels).  Host arrays of
+ * ntot doubles.  history (may be NULL): [max_cycles][nlev][4] = (rnorm, fnorm, enorm, vnorm) after
+ * each cycle, as in /cbmgdata2/ (cb.control.inc:156-164); ncycles_out: cycles run. */
+int lmgx_solve(lmgx_handle *h, double *f, double *b, double *residu, const lmgx_params *prm,
+               double *history, int *ncycles_out);
+/* Same on caller-owned device arrays (ntot doubles each, stream-ordered on the handle's stream,
+ * synchronised before return). */
+int lmgx_solve_device(lmgx_handle *h, double *d_f, double *d_b, double *d_res,
+                      const lmgx_params *prm, double *history, int *ncycles_out);
+
+/* The SAVEd sweep-order counter of code/sporlx.F:58 (persists across solves, like LESter). */
+int lmgx_get_jumble(const lmgx_handle *h);
+int lmgx_set_jumble(lmgx_handle *h, int jumble);
+
+/* After-solve steps of code/step.F:376-380: code/extrap_p.F:5-48 (zero-gradient halos on
+ * physical sides, then halo swap) and code/p_ref_subtract.F:6-100 (subtract p at cell (2,2,2)
+ * of block 1 from every point of level 1).  Device array of ntot (or n1) doubles. */
+int lmgx_extrap_p_device(lmgx_handle *h, double *d_p);
+int lmgx_p_ref_subtract_device(lmgx_handle *h, double *d_p);
+
+/* code/global_sum.F:105-147: sum, max and min of f over the interior of level l. */
+int lmgx_global_sum_device(lmgx_handle *h, int level, const double *d_f, double out[3]);
+
+/* Single passes (device arrays, level 0-based), exposed for testing and for callers that drive
+ * their own schedule: code/norms.F:5-126, code/sporlx.F:5-203 (one call = 3 sweeps),
+ * code/swap_sc0.F:6-293, code/restriction.F:5-68 (+ init_array of the coarse f),
+ * code/prolongation.F:5-75.  norms_out = (rnorm, fnorm, enorm, vnorm). */
+int lmgx_norms_device(lmgx_handle *h, int level, const double *d_x, const double *d_b,
+                      double *d_res, double norms_out[4]);
+int lmgx_sporlx_device(lmgx_handle *h, int level, double *d_f, const double *d_b, double omega);
+int lmgx_swap_device(lmgx_handle *h, int level, double *d_var);
+int lmgx_restrict_device(lmgx_handle *h, int coarse_level, const double *d_res, double *d_b,
+                         double *d_f);
+int lmgx_prolong_device(lmgx_handle *h, int fine_level, double *d_f);
+
+/* Operator set-up on the device (reference code/coeff_p_block.F:69-228 on every level and
+ * code/sour_p_block.F:89-167 on level 1, with the metric pipeline of code/mg_xyz.F,
+ * code/metric_block2.F, code/comp_metric.F in front).  xv1/yv1: level-1 interior vertices of
+ * every block ([j=2..njp1][i=2..nip1] per block, blocks in order), zv1: [k=2..nkp1] per block.
+ * rho: cell densities on level 1 incl. halos (n1 doubles, NULL = 1); when given, each face
+ * coefficient is divided by the face density 0.5*(rho_P + rho_N) and coarse levels use 8-child
+ * means.  dirichlet: [nblk][6] flags, 1 = face-Dirichlet p = 0 on that physical side (NULL =
+ * none).  The operator is stored in the handle (as lmgx_set_operator would). */
+int lmgx_build_operator(lmgx_handle *h, const double *xv1, const double *yv1, const double *zv1,
+                        const double *rho, const int *dirichlet);
+/* Copy the handle's operator to host (7 x ntot) - for checking and for hand-off to LESter. */
+int lmgx_get_operator(const lmgx_handle *h, double *ap, double *aw, double *ae, double *as,
+                      double *an, double *al, double *ar);
+/* Level-1 source b = sour_p_block(u, v, w, rho, rhonp1, dtime) into d_b (device, n1 doubles);
+ * requires a prior lmgx_build_operator (metrics). */
+int lmgx_source_device(lmgx_handle *h, const double *d_u, const double *d_v, const double *d_w,
+                       const double *d_rho, const double *d_rhonp1, double dtime, double *d_b);
+
+/* Adjoint: solve A^T lambda = g with the transposed-operator V-cycle (restriction = P^T,
+ * prolongation = R^T, backward-ordered red-black smoothing), FP64, device arrays of ntot.
+ * lambda: in = initial guess (level 1), out = solution.  Same history convention. */
+int lmgx_solve_adjoint_device(lmgx_handle *h, double *d_lambda, double *d_g, double *d_res,
+                              const lmgx_params *prm, double *history, int *ncycles_out);
+
+/* Kernel timing of the last solve: average duration (ms) and count of the dominant smoother
+ * kernel, measured with HIP events on the handle's stream. */
+int lmgx_profile(lmgx_handle *h, int enable);
+int lmgx_profile_read(const lmgx_handle *h, double *smoother_ms_total, long *smoother_launches,
+                      double *smoother_cells_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

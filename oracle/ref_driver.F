@@ -108,7 +108,9 @@ C TOTAL SIZES OF THE MULTI-LEVEL ARRAYS
      &     (mg_nijkp1(1,numbl,nummgl)+1)*(mg_nijkp1(2,numbl,nummgl)+1)
       nz = mg_off_1d_z(numbl,nummgl) + mg_nijkp1(3,numbl,nummgl) + 1
 
-      lwork = 8*numbl*(maxijk2+8) + 4096
+C     ra(jw1a) is both the swap_sc buffer and prol_block's cr scratch
+C     (prolongation.F:64-67), so it must hold a whole fine block too
+      lwork = maxijk3 + 8*numbl*(maxijk2+8) + 4096
       lrwk = lwork
       lrwk2d = lwork
       liwk = 16

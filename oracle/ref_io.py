@@ -25,11 +25,11 @@ def available() -> bool:
 def write_input(path: str, problem, *, ncycles: int, coeffs: Optional[Sequence[np.ndarray]] = None,
                 b1: Optional[np.ndarray] = None, fields: Optional[dict] = None,
                 f0: Optional[np.ndarray] = None, n1: Optional[int] = None, do_post: int = 1,
-                ntime: int = 1, tolf: float = 0.0) -> None:
+                ntime: int = 1, tolf: float = 0.0, cycle: int = 1) -> None:
     p = problem
     with open(path, "wb") as fh:
         w = fh.write
-        w(struct.pack("<12i", 4711, p.nblk, p.nummgl, 1, p.numpre, p.numpost, p.numcits, ncycles,
+        w(struct.pack("<12i", 4711, p.nblk, p.nummgl, cycle, p.numpre, p.numpost, p.numcits, ncycles,
                       0 if coeffs is None else 1, 0 if b1 is None else 1, do_post, ntime))
         w(struct.pack("<4d", p.omega, p.omega_c, tolf, p.dtime))
         for b in p.blocks:
@@ -96,13 +96,14 @@ def read_output(path: str) -> Dict[str, np.ndarray]:
 
 
 def run(problem, *, ncycles: int, coeffs=None, b1=None, fields=None, f0=None, n1=None,
-        do_post: int = 1, ntime: int = 1, tolf: float = 0.0, timeout: float = 3600.0):
+        do_post: int = 1, ntime: int = 1, tolf: float = 0.0, timeout: float = 3600.0,
+        cycle: int = 1):
     if not available():
         raise RuntimeError("reference driver not built: make -C oracle ref")
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(fin, problem, ncycles=ncycles, coeffs=coeffs, b1=b1, fields=fields, f0=f0,
-                    n1=n1, do_post=do_post, ntime=ntime, tolf=tolf)
+                    n1=n1, do_post=do_post, ntime=ntime, tolf=tolf, cycle=cycle)
         env = dict(os.environ)
         r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, env=env,
                            timeout=timeout)

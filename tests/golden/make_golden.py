@@ -1,0 +1,116 @@
+"""Writes the golden fixtures of tests/golden/ by running the reference Fortran pressure path
+(oracle/_ref/ref_driver, built from the reference sources by ``make -C oracle ref``).
+
+Run in the build container (the reference is not present on GPU machines):
+    python tests/golden/make_golden.py
+
+Each fixture holds the inputs (problem name/scale and, where the coefficients or the source are
+injected, those arrays) and the reference's outputs.  Cases:
+
+  g1_c0s4      1 block 16x16x8, 3 levels, reference coeff_p/sour_p, Neumann + periodic z, 6 V(2,2)
+  g2_c4s8      2 blocks (x-interface) + periodic z, reference coefficients, 6 V(2,2)
+  g3_c1s8      4 blocks in two x-chains, reference coefficients, 6 V(2,2)
+  g4_c0s4_f    as g1 but F-cycles, 4 cycles
+  g5_c4s8_ext  2 blocks with 1/rho face scaling and face-Dirichlet outflow (coefficients and
+               mean-free source injected), 8 V(2,2)
+  g6_c0        configs[0] at full size (64x64x32, 4 levels, Dirichlet outflow, mean-free
+               divergence source), 30 V(2,2): norm history, p after extrap_p + p_ref_subtract
+               (strided sample + checksums)
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from fortranutils_amd import problems as P  # noqa: E402
+import oracle as O  # noqa: E402
+import ref_io  # noqa: E402
+
+FULL_KEYS = ("ap", "aw", "ae", "as", "an", "al", "ar", "b", "f", "res", "hist", "p_post",
+             "x", "y", "z", "dkp", "dkw", "npts_all", "nijkp1", "mg_off")
+MET_COLS = {"diu": 1, "djv": 9, "rdiv": 5, "rdju": 12, "didjp": 29, "rdidjp": 30, "rp": 26}
+
+
+def fields(p):
+    us, vs, ws = P.velocities(p)
+    rho = P.density(p)
+    return ref_io.level1_fields(p, dict(u=us, v=vs, w=ws, rho=rho, rhonp1=rho))
+
+
+def save(name, out, extra=None, keys=FULL_KEYS):
+    d = {k: out[k] for k in keys}
+    for k, c in MET_COLS.items():
+        d["met_" + k] = out["metrics"][c]
+    if extra:
+        d.update(extra)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **d)
+    print(f"{name}: {os.path.getsize(path) / 1024:.1f} KiB")
+
+
+def ext_inputs(p):
+    """1/rho-scaled coefficients with Dirichlet outflow and a mean-free source (oracle-built)."""
+    o = O.Oracle(p)
+    geo = o.geometry()
+    fl = fields(p)
+    rho = o.rho_levels(fl["rho"])
+    dirich = np.array([b.dirichlet for b in p.blocks], np.int32)
+    a = o.coeff(geo, rho=rho, dirichlet=dirich)
+    b = o.sour(geo, fl["u"], fl["v"], fl["w"], fl["rho"], fl["rhonp1"], p.dtime)
+    b1 = mean_free(o, b[: o.n1])
+    return o, a, b1
+
+
+def mean_free(o, b1):
+    """Subtract the interior mean (halos untouched), blocks in order."""
+    b1 = b1.copy()
+    tot, cnt = 0.0, 0
+    masks = []
+    for bl in range(o.nblk):
+        n = o.nijkp1[0, bl]
+        m = np.zeros((n[2], n[1], n[0]), bool)
+        m[1:-1, 1:-1, 1:-1] = True
+        masks.append(m.ravel())
+    mask = np.concatenate(masks)
+    b1[mask] -= b1[mask].mean()
+    return b1
+
+
+def main():
+    if not ref_io.available():
+        raise SystemExit("build the reference driver first: make -C oracle ref")
+    for name, cfg, sc, ncyc, cyc in [("g1_c0s4", "c0", 4, 6, 1), ("g2_c4s8", "c4", 8, 6, 1),
+                                     ("g3_c1s8", "c1", 8, 6, 1), ("g4_c0s4_f", "c0", 4, 4, 2)]:
+        p = P.config(cfg, sc)
+        out = ref_io.run(p, ncycles=ncyc, fields=fields(p), n1=sum(int(np.prod(b.nijkp1)) for b in p.blocks),
+                         cycle=cyc)
+        save(name, out, dict(cfg=cfg, scale=sc, ncycles=ncyc, cycle=cyc))
+
+    p = P.config("c4", 8)
+    o, a, b1 = ext_inputs(p)
+    out = ref_io.run(p, ncycles=8, coeffs=a, b1=b1, n1=o.n1)
+    save("g5_c4s8_ext", out, dict(cfg="c4", scale=8, ncycles=8, cycle=1, b1_in=b1,
+                                   **{"in_" + k: v for k, v in zip("ap aw ae as an al ar".split(), a)}))
+
+    p = P.config("c0", 1)
+    o, a, b1 = ext_inputs(p)  # rho = 1 for c0: only the Dirichlet outflow differs from coeff_p
+    out = ref_io.run(p, ncycles=30, coeffs=a, b1=b1, n1=o.n1)
+    pp = out["p_post"]
+    idx = np.arange(0, pp.size, 37)
+    np.savez_compressed(os.path.join(HERE, "g6_c0.npz"), hist=out["hist"], p_idx=idx,
+                        p_sample=pp[idx], p_sum=pp.sum(), p_sumsq=(pp * pp).sum(),
+                        f_sample=out["f"][idx], tsolve=out["tsolve"], cfg="c0", scale=1,
+                        ncycles=30, cycle=1)
+    print("g6_c0:", os.path.getsize(os.path.join(HERE, "g6_c0.npz")) / 1024, "KiB",
+          "rnorm0/rnorm30 =", out["hist"][0, 0, 0] / out["hist"][-1, 0, 0])
+
+
+if __name__ == "__main__":
+    main()
