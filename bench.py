@@ -1,0 +1,246 @@
+"""Benchmark: LESter pressure-Poisson multigrid on one MI355X.
+
+Workload (default): configs[3]-shaped problem "c3" - 16 uneven blocks, 119.5M cells, nz = 128,
+tanh wall-stretched y, wall-layer density with 5% lognormal noise, 5 Gaussian source blobs on top
+of the divergence of a seeded N(0,1) velocity field, face-Dirichlet outflow, 6 levels.  Synthetic
+data, generated on the GPU before the timed region.
+
+One "step" = one complete solve from p = 0 to a 1e-10 relative residual reduction (rnorm_k /
+rnorm_0, the reference's FP64 RMS residual, code/norms.F:118).  metric value = Mcell x V-cycles
+per second over the timed steps; also reported: time to 1e-10, the same for the plain FP64 GPU
+path, the speed-up, and the pressure difference between the two.
+
+Prints one JSON line (see the README of the driver contract in DESIGN.md).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3")
+    ap.add_argument("--scale", type=int, default=1)
+    ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64", "exact"])
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--numpre", type=int, default=1)
+    ap.add_argument("--numpost", type=int, default=1)
+    ap.add_argument("--spi", type=int, default=2, help="red-black sweeps per smoothing iteration")
+    ap.add_argument("--numcits", type=int, default=20)
+    ap.add_argument("--omega", type=float, default=1.0)
+    ap.add_argument("--omega-c", type=float, default=1.0)
+    ap.add_argument("--max-cycles", type=int, default=100)
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 GPU path")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cycles", type=int, default=3)
+    return ap.parse_args()
+
+
+def make_problem(args):
+    import torch
+    from fortranutils_amd import lmgx, problems as P
+    p = P.config(args.config, args.scale)
+    s = lmgx.Solver(lmgx.Topology.from_problem(p))
+    fl = P.device_fields(p)
+    dirich = np.array([b.dirichlet for b in p.blocks], np.int32)
+    rho1 = fl["rho"].cpu().numpy() if p.rho_kind != "uniform" else None
+    s.build_operator(p, rho1=rho1, dirichlet=dirich)
+    b = torch.zeros(s.ntot, dtype=torch.float64, device="cuda")
+    s.source(fl["u"], fl["v"], fl["w"], fl["rho"], fl["rhonp1"], p.dtime, b)
+    P.device_blobs(p, b[: s.n1])
+    m = P.interior_mask_torch(p)
+    b1 = b[: s.n1]
+    b1[m] -= b1[m].mean()
+    del fl
+    torch.cuda.synchronize()
+    return p, s, b
+
+
+def solve(s, b, args, mode, rtol, profile=False):
+    import torch
+    f = torch.zeros_like(b)
+    res = torch.zeros_like(b)
+    kw = dict(tolf=0.0, rtol=rtol, max_cycles=args.max_cycles, numcits=args.numcits)
+    if mode == "exact":
+        kw.update(smoother=0, precision=0, numpre=2, numpost=2, omega=1.6, omega_c=1.6)
+    else:
+        kw.update(smoother=1, precision=1 if mode == "mixed" else 0, numpre=args.numpre,
+                  numpost=args.numpost, sweeps_per_iter=args.spi, omega=args.omega,
+                  omega_c=args.omega_c)
+    if profile:
+        s.profile(True)
+    hist = s.lmg_solve(f, b, res, **kw)
+    if profile:
+        ms, n, cells = s.profile_read()
+        s.profile(False)
+        return f, hist, (ms, n, cells)
+    return f, hist, None
+
+
+def timed(s, b, args, mode, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        solve(s, b, args, mode, args.rtol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cycles = 0
+    f = hist = None
+    for _ in range(steps):
+        f, hist, _ = solve(s, b, args, mode, args.rtol)
+        cycles += len(hist)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dt, cycles, f, hist
+
+
+def cpu_baseline(args):
+    """The reference Fortran (oracle/_ref/ref_driver, built from the reference sources) when it
+    runs here, else the C restatement: FP64 V(2,2) with the reference's point-SOR schedule on the
+    first x-chain of the same workload (2 blocks, full size), ``--cpu-cycles`` cycles, 1 core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from fortranutils_amd import problems as P
+    p = P.config(args.config, args.scale)
+    first = [0]
+    while p.blocks[first[-1]].bl2bl[1]:
+        first.append(p.blocks[first[-1]].bl2bl[1] - 1)
+    q = P.subproblem(p, first)
+    cells = q.interior_cells()
+    import ref_io
+    import oracle as O
+    o = O.Oracle(q)
+    geo = o.geometry()
+    us, vs, ws = P.velocities(q)
+    rho = P.density(q)
+    fl = dict(u=o.level1(us), v=o.level1(vs), w=o.level1(ws), rho=o.level1(rho), rhonp1=o.level1(rho))
+    dirich = np.array([b.dirichlet for b in q.blocks], np.int32)
+    a = o.coeff(geo, rho=o.rho_levels(fl["rho"]), dirichlet=dirich)
+    bb = o.sour(geo, fl["u"], fl["v"], fl["w"], fl["rho"], fl["rhonp1"], q.dtime)
+    sample = (f"{args.config} first x-chain, blocks {[i + 1 for i in first]}, {cells / 1e6:.2f}M "
+              f"cells, {o.nlev} levels, FP64 V(2,2) point-SOR (reference schedule), "
+              f"{args.cpu_cycles} cycles, 1 core")
+    if ref_io.available():
+        try:
+            out = ref_io.run(q, ncycles=args.cpu_cycles, coeffs=a, b1=bb[: o.n1], n1=o.n1,
+                             do_post=0, timeout=600)
+            return dict(value=cells * args.cpu_cycles / out["tsolve"] / 1e6,
+                        unit="Mcell*V-cycles/s", cores=1, kind="reference", sample=sample,
+                        seconds=out["tsolve"])
+        except Exception as e:  # the reference build may not run on this host
+            print("reference CPU baseline unavailable:", str(e)[:200], file=sys.stderr)
+    f = np.zeros(o.ntot)
+    t0 = time.perf_counter()
+    o.solve(a, f, bb, args.cpu_cycles)
+    dt = time.perf_counter() - t0
+    return dict(value=cells * args.cpu_cycles / dt / 1e6, unit="Mcell*V-cycles/s", cores=1,
+                kind="port", sample=sample, seconds=dt)
+
+
+def traffic_from_profiles(kernel_key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return d.get(kernel_key)
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
+              "design-only, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "needs a GPU"
+    t_setup = time.perf_counter()
+    p, s, b = make_problem(args)
+    n1cells = p.interior_cells()
+    print(f"# setup {time.perf_counter() - t_setup:.1f}s: {args.config} {n1cells / 1e6:.1f}M cells, "
+          f"{s.nlev} levels, {p.nblk} blocks", file=sys.stderr, flush=True)
+
+    dt, cycles, f, hist = timed(s, b, args, args.mode, args.steps, args.warmup)
+    ms_step = 1e3 * dt / args.steps
+    value = n1cells * cycles / dt / 1e6
+    red = hist[-1, 0, 0] / hist[0, 0, 0] if len(hist) else float("nan")
+    print(f"# {args.mode}: {cycles / args.steps:.1f} cycles/solve, {ms_step:.1f} ms/solve, "
+          f"final reduction {red:.2e}", file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_compare and args.mode != "fp64":
+        dt64, cyc64, f64, h64 = timed(s, b, args, "fp64", 1, 0)
+        err = (torch.linalg.norm(f[: s.n1] - f64[: s.n1]) / torch.linalg.norm(f64[: s.n1])).item()
+        extra = dict(fp64_ms_per_solve=1e3 * dt64, fp64_cycles=cyc64,
+                     speedup_vs_fp64=dt64 / (dt / args.steps),
+                     p_rel_l2_vs_fp64=err)
+        print(f"# fp64: {cyc64} cycles, {1e3 * dt64:.1f} ms; speed-up {extra['speedup_vs_fp64']:.2f}x,"
+              f" |p_mixed - p_fp64|/|p_fp64| = {err:.2e}", file=sys.stderr, flush=True)
+
+    # dominant kernel: the finest-level red-black half-sweep; HIP events on the solver's stream
+    roof = None
+    if args.mode != "exact":
+        _, _, (ms, nl, cells) = solve(s, b, args, args.mode, args.rtol, profile=True)
+        word = 4 if args.mode == "mixed" else 8
+        per_launch_bytes = (cells / max(nl, 1)) * 10 * word   # 10 words per updated cell
+        avg_s = ms / max(nl, 1) / 1e3
+        achieved = per_launch_bytes / avg_s
+        key = f"k_rb_half_l<{'float' if word == 4 else 'double'}, 1>:{args.config}"
+        roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
+                    frac=achieved / PEAK_HBM, traffic=traffic_from_profiles(key),
+                    kernel=key, avg_launch_us=1e6 * avg_s, launches=nl,
+                    bytes_per_launch=per_launch_bytes)
+        print(f"# smoother: {nl} launches, avg {1e6 * avg_s:.1f} us, {achieved / 1e9:.0f} GB/s",
+              file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(args)
+        except Exception as e:
+            print("cpu baseline failed:", e, file=sys.stderr)
+
+    out = {
+        "metric": "Mcell*V-cycles/s (time to 1e-10 residual reduction, mixed-prec vs FP64, 1 GPU)",
+        "value": value,
+        "unit": "Mcell*V-cycles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if args.mode == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: {p.nblk} blocks, {n1cells / 1e6:.1f}M cells, "
+                               f"{s.nlev} levels, solve to {args.rtol:g} residual reduction",
+                   "mode": args.mode, "smoother": "red-black SOR" if args.mode != "exact" else
+                   "reference lexicographic SOR", "numpre": args.numpre, "numpost": args.numpost,
+                   "sweeps_per_iter": args.spi, "omega": args.omega, "numcits": args.numcits},
+        "time_to_rtol_ms": ms_step,
+        "cycles_per_solve": cycles / args.steps,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -49,7 +49,7 @@ static inline unsigned nblocks(long n) { return (unsigned)((n + LMGX_WG - 1) / L
 struct LevelDev {
     std::vector<LevBlk> h_lb;
     LevBlk *d_lb = nullptr;
-    long tiles_int = 0, tiles_all = 0, tiles_rb = 0, nlines = 0;
+    long tiles_int = 0, tiles_all = 0, tiles_rb = 0, tiles_ln = 0, nlines = 0;
     int smax = 0;                       // largest hyperplane index i+j+k (1-based)
     std::vector<Face> h_fc;
     Face *d_fc = nullptr;
@@ -67,6 +67,7 @@ struct lmgx_handle {
     long off[LMGX_MAXLEV][LMGX_MAXB];
     double npts[LMGX_MAXLEV];
     int bc[LMGX_MAXB][6], bl2bl[LMGX_MAXB][6], bl2side[LMGX_MAXB][6];
+    int dirich[LMGX_MAXB][6];          // face-Dirichlet sides (operator extension)
     long ntot = 0, n1 = 0, n2d = 0, n1d = 0, nxy = 0, nz = 0;
     long off2d[LMGX_MAXLEV][LMGX_MAXB], off1d[LMGX_MAXLEV][LMGX_MAXB];
     long offxy[LMGX_MAXLEV][LMGX_MAXB], offz[LMGX_MAXLEV][LMGX_MAXB];
@@ -185,7 +186,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
     for (int l = 0; l < L; ++l) {
         LevelDev &D = h->lev[l];
         D.h_lb.resize(h->nblk);
-        long ti = 0, ta = 0, tr = 0, sl = 0, ns = 0;
+        long ti = 0, ta = 0, tr = 0, tl = 0, sl = 0, ns = 0;
         double np = 0.0;
         long npi = 0;
         int smax = 0;
@@ -208,6 +209,8 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             ta += (long)((n[0] + 63) / 64) * ((n[1] + 3) / 4);
             B.t_rb = tr;
             tr += (long)((B.hp + 63) / 64) * ((n[1] - 2 + 3) / 4);
+            B.t_ln = tl;
+            tl += (long)((B.hp + 63) / 64) * ((n[2] - 2 + 3) / 4);
             B.s_jk = sl;
             sl += (long)(n[1] - 2) * (n[2] - 2);
             B.o2 = h->off2d[l][b];
@@ -221,6 +224,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
         D.tiles_int = ti;
         D.tiles_all = ta;
         D.tiles_rb = tr;
+        D.tiles_ln = tl;
         D.nlines = sl;
         D.smax = smax;
         D.npts_int = npi;
@@ -292,7 +296,8 @@ static int upload_tables(lmgx_handle *h)
             HIPCHK(hipMemcpy(D.d_fc, D.h_fc.data(), sizeof(Face) * D.nfaces, hipMemcpyHostToDevice));
         }
         swap_need = std::max(swap_need, D.nel);
-        part_need = std::max(part_need, 4 * std::max(D.tiles_int, std::max(D.tiles_all, D.tiles_rb)));
+        part_need = std::max(part_need, 4 * std::max(std::max(D.tiles_int, D.tiles_ln),
+                                                     std::max(D.tiles_all, D.tiles_rb)));
     }
     HIPCHK(hipMalloc(&h->d_swap, sizeof(double) * swap_need));
     h->swap_cap = swap_need;
@@ -318,6 +323,8 @@ extern "C" void lmgx_default_params(lmgx_params *p)
     p->precision = LMGX_PREC_FP64;
     p->check_every = 1;
     p->sweeps_per_iter = 3;
+    p->coarse_op = 1;
+    p->coarse_solve = 1;
 }
 
 static void free_fast(lmgx_handle *h);
@@ -378,6 +385,20 @@ extern "C" int lmgx_levels(const lmgx_handle *h, int *nlev, long *ntot, long *n1
                     nijkp1_all[(l * h->nblk + b) * 3 + d] = h->nijkp1[l][b][d];
             if (off_all) off_all[l * h->nblk + b] = h->off[l][b];
         }
+    return 0;
+}
+
+extern "C" int lmgx_set_dirichlet(lmgx_handle *h, const int *flags)
+{
+    if (!h) return fail(LMGX_E_ARG, "NULL handle");
+    for (int b = 0; b < h->nblk; ++b)
+        for (int s = 0; s < 6; ++s) {
+            h->dirich[b][s] = flags ? (flags[6 * b + s] != 0 && !is_swap(h->bc[b][s])) : 0;
+            for (int l = 0; l < h->nlev; ++l) h->lev[l].h_lb[b].dir[s] = h->dirich[b][s];
+        }
+    for (int l = 0; l < h->nlev; ++l)
+        HIPCHK(hipMemcpy(h->lev[l].d_lb, h->lev[l].h_lb.data(), sizeof(LevBlk) * h->nblk,
+                         hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -18,9 +18,11 @@ struct LevBlk {
     int hp;          // slots per (j,k) line per colour in the split layout = ceil(nip1/2)
     int ifc[6];      // side is an INTERFACE (correction range covers the halo)
     int swp[6];      // side is INTERFACE or PERIODIC (halo swapped)
+    int dir[6];      // side carries a face-Dirichlet p = 0 (fast-path prolongation: odd extension)
     long t_int;      // first tile of this block in the interior (i,j) tiling
     long t_all;      // first tile of this block in the full (i,j) tiling (halos included)
     long t_rb;       // first tile of this block in the split-layout (slot,j) tiling
+    long t_ln;       // first tile of this block in the split-layout (slot,k) line tiling
     long s_jk;       // first (j,k) interior line of this block (hyperplane sweep)
     long o2;         // offset of the block's 2-D (i,j) metric arrays
     long o1;         // offset of the block's 1-D k metric arrays
@@ -61,7 +63,7 @@ __device__ inline int find_block(const LevBlk *lb, int nblk, long t, int which)
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         long s = which == 0 ? lb[mid].t_int : which == 1 ? lb[mid].t_all : which == 2 ? lb[mid].t_rb
-                                                                                       : lb[mid].s_jk;
+                 : which == 3 ? lb[mid].s_jk : lb[mid].t_ln;
         if (s <= t) lo = mid; else hi = mid - 1;
     }
     return lo;

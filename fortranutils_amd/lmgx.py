@@ -36,7 +36,7 @@ class Params(C.Structure):
                 ("numcits", C.c_int), ("max_cycles", C.c_int), ("omega", C.c_double),
                 ("omega_c", C.c_double), ("tolf", C.c_double), ("rtol", C.c_double),
                 ("smoother", C.c_int), ("precision", C.c_int), ("check_every", C.c_int),
-                ("sweeps_per_iter", C.c_int)]
+                ("sweeps_per_iter", C.c_int), ("coarse_op", C.c_int), ("coarse_solve", C.c_int)]
 
 
 class _Topo(C.Structure):
@@ -54,6 +54,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise LmgxError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        # PyTorch ships its own HIP runtime with the same SONAME (libamdhip64.so.7).  Loading it
+        # first makes liblmgx bind to that one runtime, so torch tensors and liblmgx share a
+        # single HIP context; without torch, liblmgx uses /opt/rocm's runtime.
+        try:
+            import torch  # noqa: F401
+            import torch.cuda  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         L.lmgx_last_error.restype = C.c_char_p
         _lib = L
@@ -160,6 +168,10 @@ class Solver:
         _check(lib().lmgx_build_operator(self.h, _dp(xv), _dp(yv), _dp(zv),
                                          None if r is None else _dp(r),
                                          None if d is None else _ip(d)))
+
+    def set_dirichlet(self, flags):
+        f = None if flags is None else np.ascontiguousarray(flags, np.int32).ravel()
+        _check(lib().lmgx_set_dirichlet(self.h, None if f is None else _ip(f)))
 
     def get_operator(self):
         a = [np.zeros(self.ntot) for _ in range(7)]

@@ -115,9 +115,11 @@ def config(name: str, scale: int = 1) -> Problem:
 
     c0  1 block 64x64x32, tanh-stretched y, rho = 1, 4 levels, V(2,2)
     c1  4 uneven blocks (192/128/96/64 x 96/96/48/48 x 64) in two x-chains, mixing layer rho
-    c2  1 block 512x256x256, mixing layer, 5 source blobs, 6 levels
+    c2  1 block 512x256x256 on the 8x2x4 box, mixing layer, 5 source blobs, 6 levels
     c3  16 uneven blocks (~120M cells), nz = 128, wall-layer rho with lognormal noise, 6 levels
     c4  2 blocks 128x96x64 + 96x96x64 (one x-interface), mixing layer, 4 levels
+    c0 keeps its 8x2x4 box; c1, c3, c4 use dx = dz (z in [0, 4]) with x extents from the cell
+    counts, so cells are isotropic in x-z and the tanh-stretched y is the stiff direction.
     """
     s = scale
     if name == "c0":
@@ -130,10 +132,11 @@ def config(name: str, scale: int = 1) -> Problem:
         zv = np.linspace(0.0, 4.0, 64 // s + 1)
         ya = tanh_vertices(96 // s, -1.0, 1.0)
         yb = tanh_vertices(48 // s, -1.0, 1.0)
+        h = 4.0 / 64  # dx = dz: x-z isotropic cells, the wall-normal y is the stiff direction
         blocks = _chain_x([(192 // s, 96 // s, 64 // s), (128 // s, 96 // s, 64 // s)],
-                          0.0, 8.0, ya, zv, 1)
+                          0.0, 320 * h, ya, zv, 1)
         blocks += _chain_x([(96 // s, 48 // s, 64 // s), (64 // s, 48 // s, 64 // s)],
-                           0.0, 8.0, yb, zv, 3)
+                           0.0, 160 * h, yb, zv, 3)
         return Problem("c1", blocks, nummgl=4 if s == 1 else 0, rho_kind="mixing")
     if name == "c2":
         nx, ny, nz = 512 // s, 256 // s, 256 // s
@@ -146,8 +149,9 @@ def config(name: str, scale: int = 1) -> Problem:
     if name == "c4":
         zv = np.linspace(0.0, 4.0, 64 // s + 1)
         yv = tanh_vertices(96 // s, -1.0, 1.0)
+        h = 4.0 / 64
         blocks = _chain_x([(128 // s, 96 // s, 64 // s), (96 // s, 96 // s, 64 // s)],
-                          0.0, 8.0, yv, zv, 1)
+                          0.0, 224 * h, yv, zv, 1)
         return Problem("c4", blocks, nummgl=4 if s == 1 else 0, rho_kind="mixing")
     raise ValueError(f"unknown config {name!r}")
 
@@ -177,7 +181,8 @@ def _config3(s: int) -> Problem:
     for ny in sorted(groups):
         chain = [(nx // s, ny // s, nz // s) for (nx, ny_, nz) in groups[ny]]
         yv = wall_vertices(ny // s, 0.0, 2.0)
-        blocks += _chain_x(chain, 0.0, 8.0, yv, zv, len(blocks) + 1)
+        h = 4.0 / 128  # dx = dz
+        blocks += _chain_x(chain, 0.0, h * sum(c[0] for c in groups[ny]), yv, zv, len(blocks) + 1)
     return Problem("c3", blocks, nummgl=6 if s == 1 else 0, rho_kind="wall", blobs=5)
 
 
@@ -253,3 +258,96 @@ def blob_source(p: Problem, amp: float = 10.0, width: float = 0.05) -> List[np.n
             g += amp * np.exp(-r2 / (2.0 * width * width))
         out.append(g * vol)
     return out
+
+
+def subproblem(p: Problem, blocks: Sequence[int], name: Optional[str] = None) -> Problem:
+    """The given 0-based blocks (closed under their links) as a problem of their own."""
+    import copy
+    keep = list(blocks)
+    remap = {b + 1: i + 1 for i, b in enumerate(keep)}
+    out = []
+    for b in keep:
+        blk = copy.deepcopy(p.blocks[b])
+        for s in range(6):
+            if blk.bl2bl[s]:
+                if blk.bl2bl[s] not in remap:
+                    raise ValueError("sub-problem is not closed under its links")
+                blk.bl2bl[s] = remap[blk.bl2bl[s]]
+        out.append(blk)
+    q = copy.copy(p)
+    q.blocks = out
+    q.name = name or (p.name + "_sub")
+    return q
+
+
+def device_fields(p: Problem, device="cuda", seed_offset: int = 0):
+    """Level-1 fields (concatenated blocks, [k,j,i] per block) generated on the GPU with torch:
+    u, v, w ~ N(0,1) (seeded), rho per ``p.rho_kind`` (lognormal noise seeded), rhonp1 = rho."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(p.src_seed + seed_offset)
+    n1 = sum(int(np.prod(b.nijkp1)) for b in p.blocks)
+    u = torch.randn(n1, generator=g, device=device, dtype=torch.float64)
+    v = torch.randn(n1, generator=g, device=device, dtype=torch.float64)
+    w = torch.randn(n1, generator=g, device=device, dtype=torch.float64)
+    rho = torch.empty(n1, device=device, dtype=torch.float64)
+    o = 0
+    for b in p.blocks:
+        nip1, njp1, nkp1 = b.nijkp1
+        n = nip1 * njp1 * nkp1
+        _, yc, _ = cell_centres(b)
+        if p.rho_kind == "uniform":
+            ry = np.ones_like(yc)
+        elif p.rho_kind == "mixing":
+            ry = 1.0 + 3.0 * (1.0 + np.tanh(yc / 0.1))
+        else:
+            ry = 1.0 + 2.0 * np.exp(-np.maximum(yc, 0.0) / 0.05)
+        r = torch.from_numpy(ry).to(device)[None, :, None].expand(nkp1, njp1, nip1).reshape(-1)
+        if p.rho_kind == "wall":
+            noise = torch.exp(0.05 * torch.randn(n, generator=g, device=device, dtype=torch.float64))
+            r = r * noise
+        rho[o:o + n] = r
+        o += n
+    return dict(u=u, v=v, w=w, rho=rho, rhonp1=rho)
+
+
+def device_blobs(p: Problem, b1, device="cuda", amp: float = 10.0, width: float = 0.05):
+    """Add ``p.blobs`` Gaussian blobs (seeded centres) x cell volume to the level-1 source b1."""
+    import torch
+    if p.blobs == 0:
+        return
+    rng = np.random.default_rng(p.src_seed + 5)
+    xs = np.concatenate([b.xv for b in p.blocks])
+    ys = np.concatenate([b.yv for b in p.blocks])
+    zs = np.concatenate([b.zv for b in p.blocks])
+    cen = np.stack([rng.uniform(xs.min(), xs.max(), p.blobs),
+                    rng.uniform(ys.min(), ys.max(), p.blobs),
+                    rng.uniform(zs.min(), zs.max(), p.blobs)], 1)
+    o = 0
+    for b in p.blocks:
+        nip1, njp1, nkp1 = b.nijkp1
+        n = nip1 * njp1 * nkp1
+        xc, yc, zc = [torch.from_numpy(a).to(device) for a in cell_centres(b)]
+        dx = torch.from_numpy(np.diff(_ghosted(b.xv))).to(device)
+        dy = torch.from_numpy(np.diff(_ghosted(b.yv))).to(device)
+        dz = torch.from_numpy(np.diff(_ghosted(b.zv))).to(device)
+        vol = dz[:, None, None] * dy[None, :, None] * dx[None, None, :]
+        gsum = torch.zeros((nkp1, njp1, nip1), device=device, dtype=torch.float64)
+        for c in cen:
+            r2 = ((xc[None, None, :] - c[0]) ** 2 + (yc[None, :, None] - c[1]) ** 2
+                  + (zc[:, None, None] - c[2]) ** 2)
+            gsum += amp * torch.exp(-r2 / (2.0 * width * width))
+        blk = b1[o:o + n].view(nkp1, njp1, nip1)
+        blk[1:-1, 1:-1, 1:-1] += (gsum * vol)[1:-1, 1:-1, 1:-1]
+        o += n
+
+
+def interior_mask_torch(p: Problem, device="cuda"):
+    import torch
+    ms = []
+    for b in p.blocks:
+        nip1, njp1, nkp1 = b.nijkp1
+        m = torch.zeros((nkp1, njp1, nip1), dtype=torch.bool, device=device)
+        m[1:-1, 1:-1, 1:-1] = True
+        ms.append(m.reshape(-1))
+    return torch.cat(ms)

@@ -44,10 +44,10 @@ typedef struct lmgx_topology {
 
 /* smoother */
 #define LMGX_SMOOTH_EXACT 0   /* reference order: 3 lexicographic SOR sweeps per iteration */
-#define LMGX_SMOOTH_RB 1      /* red-black SOR, colour-split layout */
+#define LMGX_SMOOTH_RB 1      /* zebra y-line relaxation (lines coloured by i+k), colour-split layout */
 /* precision */
 #define LMGX_PREC_FP64 0
-#define LMGX_PREC_MIXED 1     /* FP32 V-cycle inside FP64 iterative refinement */
+#define LMGX_PREC_MIXED 1     /* FP32 V-cycle inside FP64 iterative refinement (needs SMOOTH_RB) */
 
 typedef struct lmgx_params {
     int cycle;            /* 1 = V-cycle (meth_cycle), 2 = F-cycle (exact smoother only) */
@@ -61,7 +61,11 @@ typedef struct lmgx_params {
     int smoother;         /* LMGX_SMOOTH_* */
     int precision;        /* LMGX_PREC_* (MIXED needs LMGX_SMOOTH_RB) */
     int check_every;      /* fast path: test convergence every N cycles (>= 1) */
-    int sweeps_per_iter;  /* RB smoother: RB sweeps per smoothing iteration (reference: 3) */
+    int sweeps_per_iter;  /* fast path: zebra line sweeps per smoothing iteration (reference SOR: 3) */
+    int coarse_op;        /* fast path: 0 = coarse levels as given (re-discretised, as coeff_p),
+                             1 = built from level 1 (half the piecewise-constant Galerkin product) */
+    int coarse_solve;     /* fast path: 0 = numcits relaxations on the coarsest level (solversc),
+                             1 = exact solve (Cholesky inverse) when the coarsest operator is SPD */
 } lmgx_params;
 
 /* Default parameters: the reference's hard-wired knobs (code/defaults.F:59-71). */
@@ -100,6 +104,13 @@ int lmgx_solve(lmgx_handle *h, double *f, double *b, double *residu, const lmgx_
  * synchronised before return). */
 int lmgx_solve_device(lmgx_handle *h, double *d_f, double *d_b, double *d_res,
                       const lmgx_params *prm, double *history, int *ncycles_out);
+
+/* Face-Dirichlet sides ([nblk][6] flags, NULL = none) of an operator given through
+ * lmgx_set_operator (lmgx_build_operator records them itself).  The reference has none (every
+ * physical side is Neumann, code/coeff_p_block.F:160-228); the fast path uses them to extend the
+ * coarse correction oddly (face value 0) instead of the reference's zero-gradient copy
+ * (code/prol_block.F:103-135).  The reference-order path ignores them. */
+int lmgx_set_dirichlet(lmgx_handle *h, const int *flags);
 
 /* The SAVEd sweep-order counter of code/sporlx.F:58 (persists across solves, like LESter). */
 int lmgx_get_jumble(const lmgx_handle *h);

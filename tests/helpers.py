@@ -72,3 +72,62 @@ def level_slice(o, level):
     a = o.off[level, 0]
     n = o.nijkp1[level]
     return slice(int(a), int(a + np.sum(np.prod(n, axis=1))))
+
+
+def assemble_level1(o, coeffs):
+    """Sparse matrix A (interior unknowns of level 1) with A x = b equivalent to the stencil
+    r = b - ap x + sum a_nb x_nb = 0, halos of swapped sides replaced by the partner's cell they
+    copy (swap_sc), physical halos dropped (their coefficients are zero).  Returns (A, index map
+    from level-1 array position to unknown number, -1 for halos)."""
+    import scipy.sparse as sp
+    p = o.problem
+    n1 = o.n1
+    idx = -np.ones(n1, np.int64)
+    mask = interior_mask(o, 0)
+    idx[mask] = np.arange(mask.sum())
+    dims = o.nijkp1[0]
+    offs = o.off[0]
+
+    def pos(b, i, j, k):  # 0-based natural
+        n = dims[b]
+        return int(offs[b] + i + n[0] * (j + n[1] * k))
+
+    def resolve(b, i, j, k):
+        """Follow a halo point to the interior cell it holds after a swap."""
+        n = dims[b]
+        c = [i, j, k]
+        for ax in range(3):
+            for s, hv in ((2 * ax + 1, 0), (2 * ax + 2, n[ax] - 1)):
+                if c[ax] == hv:
+                    blk = p.blocks[b]
+                    if blk.bc[s - 1] not in (12, 13):
+                        return None
+                    nb, ns = blk.bl2bl[s - 1] - 1, blk.bl2side[s - 1]
+                    nn = dims[nb]
+                    ax2 = (ns - 1) // 2
+                    assert ax2 == ax, "only same-axis links"
+                    cc = list(c)
+                    cc[ax] = 1 if ns % 2 == 1 else nn[ax] - 2
+                    return resolve(nb, *cc)
+        return pos(b, *c)
+
+    rows, cols, vals = [], [], []
+    ap, aw, ae, as_, an, al, ar = coeffs
+    for b in range(o.nblk):
+        n = dims[b]
+        for k in range(1, n[2] - 1):
+            for j in range(1, n[1] - 1):
+                for i in range(1, n[0] - 1):
+                    q = pos(b, i, j, k)
+                    r = idx[q]
+                    rows.append(r); cols.append(r); vals.append(ap[q])
+                    for a, (di, dj, dk) in ((aw, (-1, 0, 0)), (ae, (1, 0, 0)), (as_, (0, -1, 0)),
+                                            (an, (0, 1, 0)), (al, (0, 0, -1)), (ar, (0, 0, 1))):
+                        if a[q] == 0.0:
+                            continue
+                        t = resolve(b, i + di, j + dj, k + dk)
+                        assert t is not None, "non-zero coefficient into a physical halo"
+                        rows.append(r); cols.append(idx[t]); vals.append(-a[q])
+    N = int(mask.sum())
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(N, N))
+    return A, idx

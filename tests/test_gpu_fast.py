@@ -1,0 +1,53 @@
+"""Zebra y-line fast path (FP64 and mixed FP32/FP64) on problems with a face-Dirichlet outflow
+(non-singular), checked against a scipy.sparse direct solve of the same level-1 operator.
+Tolerances: relative L2 of the pressure 1e-9 (FP64 and mixed, at 1e-11 residual reduction)."""
+import numpy as np
+import pytest
+
+from helpers import assemble_level1, fields, interior_mask
+from fortranutils_amd import problems as P
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(cfg, scale):
+    import torch
+    from fortranutils_amd import lmgx
+    p = P.config(cfg, scale)
+    o = O.Oracle(p)
+    s = lmgx.Solver(lmgx.Topology.from_problem(p))
+    fl = fields(p, o)
+    dirich = np.array([b.dirichlet for b in p.blocks], np.int32)
+    rho = fl["rho"] if p.rho_kind != "uniform" else None
+    s.build_operator(p, rho1=rho, dirichlet=dirich)
+    coeffs = s.get_operator()
+    b = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    dev = lambda a: torch.from_numpy(a).cuda()
+    s.source(dev(fl["u"]), dev(fl["v"]), dev(fl["w"]), dev(fl["rho"]), dev(fl["rhonp1"]),
+             p.dtime, b)
+    return p, o, s, coeffs, b
+
+
+@pytest.mark.parametrize("cfg,scale", [("c0", 2), ("c1", 4), ("c4", 4)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fast_converges_to_direct_solution(cfg, scale, prec):
+    import torch
+    import scipy.sparse.linalg as spla
+    p, o, s, coeffs, b = setup(cfg, scale)
+    A, idx = assemble_level1(o, coeffs)
+    bh = b.cpu().numpy()[: o.n1]
+    m = interior_mask(o, 0)
+    x_ref = spla.spsolve(A.tocsc(), bh[m])
+    f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    res = torch.zeros_like(f)
+    hist = s.lmg_solve(f, b, res, max_cycles=60, tolf=0.0, rtol=1e-11, smoother=1, precision=prec,
+                       numpre=1, numpost=1, omega=1.0, omega_c=1.0, sweeps_per_iter=1)
+    print(cfg, scale, prec, "cycles", len(hist), "rnorm", hist[-1, 0, 0] / hist[0, 0, 0])
+    assert len(hist) < 60
+    x = f.cpu().numpy()[: o.n1][m]
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert err < 1e-9, err
+    # the returned level-1 residual is the residual of the returned solution
+    r = res.cpu().numpy()[: o.n1][m]
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(bh[m])
