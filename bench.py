@@ -48,7 +48,20 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 GPU path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     return ap.parse_args()
+
+
+def copy_calib():
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calib_bytes": 4 * n, "dispatches": 3}))
 
 
 def make_problem(args):
@@ -163,6 +176,9 @@ def traffic_from_profiles(kernel_key):
 
 def main():
     args = parse()
+    if args.copy_calib:
+        copy_calib()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
               "design-only, see DESIGN.md", file=sys.stderr)
@@ -205,8 +221,8 @@ def main():
                     frac=achieved / PEAK_HBM, traffic=traffic_from_profiles(key),
                     kernel=key, avg_launch_us=1e6 * avg_s, launches=nl,
                     bytes_per_launch=per_launch_bytes)
-        print(f"# smoother: {nl} launches, avg {1e6 * avg_s:.1f} us, {achieved / 1e9:.0f} GB/s",
-              file=sys.stderr, flush=True)
+        print(f"# smoother: {nl} launches, avg {1e6 * avg_s:.1f} us, {achieved / 1e9:.0f} GB/s, "
+              f"tile widths fp64/fp32 {s.tile_widths().tolist()}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -50,6 +50,7 @@ struct LevelDev {
     std::vector<LevBlk> h_lb;
     LevBlk *d_lb = nullptr;
     long tiles_int = 0, tiles_all = 0, tiles_rb = 0, tiles_ln = 0, nlines = 0;
+    long tiles_l2 = 0, tiles_l4 = 0;
     int smax = 0;                       // largest hyperplane index i+j+k (1-based)
     std::vector<Face> h_fc;
     Face *d_fc = nullptr;
@@ -186,7 +187,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
     for (int l = 0; l < L; ++l) {
         LevelDev &D = h->lev[l];
         D.h_lb.resize(h->nblk);
-        long ti = 0, ta = 0, tr = 0, tl = 0, sl = 0, ns = 0;
+        long ti = 0, ta = 0, tr = 0, tl = 0, tl2 = 0, tl4 = 0, sl = 0, ns = 0;
         double np = 0.0;
         long npi = 0;
         int smax = 0;
@@ -211,6 +212,10 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             tr += (long)((B.hp + 63) / 64) * ((n[1] - 2 + 3) / 4);
             B.t_ln = tl;
             tl += (long)((B.hp + 63) / 64) * ((n[2] - 2 + 3) / 4);
+            B.t_l2 = tl2;
+            tl2 += (long)((B.hp + 127) / 128) * ((n[2] - 2 + 1) / 2);
+            B.t_l4 = tl4;
+            tl4 += (long)((B.hp + 255) / 256) * (n[2] - 2);
             B.s_jk = sl;
             sl += (long)(n[1] - 2) * (n[2] - 2);
             B.o2 = h->off2d[l][b];
@@ -225,6 +230,8 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
         D.tiles_all = ta;
         D.tiles_rb = tr;
         D.tiles_ln = tl;
+        D.tiles_l2 = tl2;
+        D.tiles_l4 = tl4;
         D.nlines = sl;
         D.smax = smax;
         D.npts_int = npi;

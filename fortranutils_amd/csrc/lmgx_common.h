@@ -22,7 +22,9 @@ struct LevBlk {
     long t_int;      // first tile of this block in the interior (i,j) tiling
     long t_all;      // first tile of this block in the full (i,j) tiling (halos included)
     long t_rb;       // first tile of this block in the split-layout (slot,j) tiling
-    long t_ln;       // first tile of this block in the split-layout (slot,k) line tiling
+    long t_ln;       // first tile of this block in the split-layout (slot,k) line tiling 64x4
+    long t_l2;       // same, 128 slots x 2 k-planes per workgroup
+    long t_l4;       // same, 256 slots x 1 k-plane per workgroup
     long s_jk;       // first (j,k) interior line of this block (hyperplane sweep)
     long o2;         // offset of the block's 2-D (i,j) metric arrays
     long o1;         // offset of the block's 1-D k metric arrays
@@ -63,7 +65,8 @@ __device__ inline int find_block(const LevBlk *lb, int nblk, long t, int which)
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         long s = which == 0 ? lb[mid].t_int : which == 1 ? lb[mid].t_all : which == 2 ? lb[mid].t_rb
-                 : which == 3 ? lb[mid].s_jk : lb[mid].t_ln;
+                 : which == 3 ? lb[mid].s_jk : which == 4 ? lb[mid].t_ln
+                 : which == 5 ? lb[mid].t_l2 : lb[mid].t_l4;
         if (s <= t) lo = mid; else hi = mid - 1;
     }
     return lo;

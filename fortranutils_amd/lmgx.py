@@ -243,6 +243,11 @@ class Solver:
         _check(lib().lmgx_global_sum_device(self.h, level, _dp(f), _dp(out)))
         return out
 
+    def tile_widths(self):
+        out = np.zeros(2 * MAXL, np.int32)
+        _check(lib().lmgx_tile_widths(self.h, _ip(out)))
+        return out.reshape(2, MAXL)[:, : self.nlev]
+
     # ---- profiling ----------------------------------------------------------------------
     def profile(self, enable=True):
         _check(lib().lmgx_profile(self.h, int(enable)))

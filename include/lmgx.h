@@ -161,6 +161,10 @@ int lmgx_source_device(lmgx_handle *h, const double *d_u, const double *d_v, con
 int lmgx_solve_adjoint_device(lmgx_handle *h, double *d_lambda, double *d_g, double *d_res,
                               const lmgx_params *prm, double *history, int *ncycles_out);
 
+/* Line-tile widths chosen by the fast path's autotuner: out[precision][LMGX_MAXL], precision 0 =
+ * FP64, 1 = FP32; 0 for levels that do not exist or before the first fast solve. */
+int lmgx_tile_widths(const lmgx_handle *h, int *out);
+
 /* Kernel timing of the last solve: average duration (ms) and count of the dominant smoother
  * kernel, measured with HIP events on the handle's stream. */
 int lmgx_profile(lmgx_handle *h, int enable);
