@@ -1,6 +1,6 @@
 """Benchmark: LESter pressure-Poisson multigrid on one MI355X.
 
-Workload (default): configs[3]-shaped problem "c3" - 16 uneven blocks, 119.5M cells, nz = 128,
+Workload (default): problem "c3" - 16 uneven blocks (seeded nx 128..640, ny 64..256), 119.5M cells, nz = 128,
 tanh wall-stretched y, wall-layer density with 5% lognormal noise, 5 Gaussian source blobs on top
 of the divergence of a seeded N(0,1) velocity field, face-Dirichlet outflow, 6 levels.  Synthetic
 data, generated on the GPU before the timed region.
@@ -10,7 +10,14 @@ rnorm_0, the reference's FP64 RMS residual, code/norms.F:118).  metric value = M
 per second over the timed steps; also reported: time to 1e-10, the same for the plain FP64 GPU
 path, the speed-up, and the pressure difference between the two.
 
-Prints one JSON line (see the README of the driver contract in DESIGN.md).
+Prints one JSON line: metric, value (Mcell x V-cycles/s of the mixed-precision solve), the
+time to 1e-10, the plain FP64 comparison, the roofline of the dominant kernel (the finest-level
+FP32 line-relaxation half-sweep, HIP events on the solver's stream) and the CPU baseline.
+
+Profiling helpers (run on rocprofv3 output):
+  --pmc-summary CSV [CSV ...]   per-kernel dispatch count and average counter value
+  --traffic DIR                 HBM bytes per launch of the finest line kernel from FETCH_SIZE /
+                                WRITE_SIZE runs of this benchmark and of --copy-calib
 """
 from __future__ import annotations
 
@@ -40,7 +47,7 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--numpre", type=int, default=1)
     ap.add_argument("--numpost", type=int, default=1)
-    ap.add_argument("--spi", type=int, default=2, help="red-black sweeps per smoothing iteration")
+    ap.add_argument("--spi", type=int, default=2, help="line-relaxation sweeps per smoothing iteration")
     ap.add_argument("--numcits", type=int, default=20)
     ap.add_argument("--omega", type=float, default=1.0)
     ap.add_argument("--omega-c", type=float, default=1.0)
@@ -50,6 +57,8 @@ def parse():
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="CSV")
+    ap.add_argument("--traffic", metavar="DIR")
     return ap.parse_args()
 
 
@@ -163,15 +172,82 @@ def cpu_baseline(args):
                 kind="port", sample=sample, seconds=dt)
 
 
-def traffic_from_profiles(kernel_key):
+def traffic_from_profiles(workload, kernel):
+    """HBM bytes per launch of ``kernel`` on ``workload`` from profiles/traffic.json (written by
+    --traffic from rocprofv3 FETCH_SIZE / WRITE_SIZE runs); None when not measured."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
-        d = json.load(open(path))
-        return d.get(kernel_key)
-    except Exception:
+        d = json.load(open(path)).get(workload)
+    except (OSError, ValueError):
         return None
+    if not d or kernel not in d:
+        return None
+    return d[kernel].get("bytes_per_launch")
+
+
+def _counter_rows(path):
+    import csv
+    for r in csv.DictReader(open(path)):
+        name = r.get("Kernel_Name") or r.get("Kernel-Name") or "?"
+        yield name, float(r.get("Counter_Value") or r.get("Counter-Value") or 0.0)
+
+
+def pmc_summary(paths):
+    """Per-kernel dispatch count / total / average of a rocprofv3 counter_collection.csv."""
+    import collections
+    for path in paths:
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        for name, v in _counter_rows(path):
+            agg[name][0] += 1
+            agg[name][1] += v
+        print(f"# {path}")
+        print("kernel,dispatches,total,avg_per_dispatch")
+        for k, (n, t) in sorted(agg.items(), key=lambda x: -x[1][1]):
+            print(f'"{k[:160]}",{n},{t:.1f},{t / n:.1f}')
+
+
+def traffic(d):
+    """profiles/traffic.json entry from four rocprofv3 --pmc runs under directory ``d``:
+    copy_FETCH_SIZE/, copy_WRITE_SIZE/ (``--copy-calib``: 3 copies of 1 GiB) and
+    <config>_<mode>_FETCH_SIZE/, _WRITE_SIZE/ (one solve).  Counters are in KiB per dispatch;
+    each is scaled so the copy kernel reads / writes exactly 1 GiB."""
+    import glob
+    def vals(sub, pred):
+        f = glob.glob(os.path.join(d, sub, "**", "*counter_collection.csv"), recursive=True)
+        return [v for name, v in _counter_rows(f[0]) if pred(name)]
+
+    def avg(sub, pred):
+        v = vals(sub, pred)
+        return (sum(v) / len(v) if v else float("nan")), len(v)
+    # the three 1 GiB copies, however the runtime splits them into dispatches
+    big = lambda name: "copy" in name.lower()
+    cf = sum(vals("copy_FETCH_SIZE", big)) / 3
+    cw = sum(vals("copy_WRITE_SIZE", big)) / 3
+    kf_scale, kw_scale = (1 << 30) / (cf * 1024), (1 << 30) / (cw * 1024)
+    out = {}
+    for sub in sorted(glob.glob(os.path.join(d, "*_FETCH_SIZE"))):
+        wl = os.path.basename(sub)[: -len("_FETCH_SIZE")]
+        if wl == "copy":
+            continue
+        cfg, mode = wl.split("_", 1)
+        word = "float" if mode == "mixed" else "double"
+        fetch_rows = list(_counter_rows(glob.glob(os.path.join(sub, "**", "*counter_collection.csv"),
+                                                  recursive=True)[0]))
+        # every finest-level line-kernel instance (tile width, relaxation) separately
+        names = sorted({n.replace("void ", "").split("(")[0] for n, _ in fetch_rows
+                        if n.replace("void ", "").startswith(f"k_zebra_line<{word}, 1,")})
+        ent = {}
+        for kname in names:
+            pick = lambda name, k=kname: name.replace("void ", "").split("(")[0] == k
+            fetch, nf = avg(os.path.basename(sub), pick)
+            write, nw = avg(wl + "_WRITE_SIZE", pick)
+            ent[kname] = dict(dispatches=nf, fetch_kib_raw=fetch, write_kib_raw=write,
+                              fetch_scale=kf_scale, write_scale=kw_scale,
+                              bytes_per_launch=fetch * 1024 * kf_scale + write * 1024 * kw_scale)
+        out[f"{cfg}:{mode}"] = ent
+    print(json.dumps(out, indent=1))
 
 
 def main():
@@ -179,9 +255,15 @@ def main():
     if args.copy_calib:
         copy_calib()
         return
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary)
+        return
+    if args.traffic:
+        traffic(args.traffic)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
-              "design-only, see DESIGN.md", file=sys.stderr)
+              "not built: every block of the problem is resident on one GPU", file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), "needs a GPU"
@@ -208,19 +290,27 @@ def main():
         print(f"# fp64: {cyc64} cycles, {1e3 * dt64:.1f} ms; speed-up {extra['speedup_vs_fp64']:.2f}x,"
               f" |p_mixed - p_fp64|/|p_fp64| = {err:.2e}", file=sys.stderr, flush=True)
 
-    # dominant kernel: the finest-level red-black half-sweep; HIP events on the solver's stream
+    # dominant kernel: the finest-level line-relaxation half-sweep; HIP events on the solver's
+    # stream.  Algorithmic bytes per updated cell: 8 operator words (b, aw, ae, al, ar, as|an,
+    # inv, cp), 1 word of the other colour's x, 1 word written (x) - 10 words; 11 when
+    # over-relaxed (the old x is read).
     roof = None
     if args.mode != "exact":
         _, _, (ms, nl, cells) = solve(s, b, args, args.mode, args.rtol, profile=True)
         word = 4 if args.mode == "mixed" else 8
-        per_launch_bytes = (cells / max(nl, 1)) * 10 * word   # 10 words per updated cell
+        words = 10 if args.omega == 1.0 else 11
+        per_launch_bytes = (cells / max(nl, 1)) * words * word
         avg_s = ms / max(nl, 1) / 1e3
         achieved = per_launch_bytes / avg_s
-        key = f"k_rb_half_l<{'float' if word == 4 else 'double'}, 1>:{args.config}"
+        tm = int(s.tile_widths()[1 if word == 4 else 0][0])
+        key = (f"k_zebra_line<{'float' if word == 4 else 'double'}, 1, {tm}, "
+               f"{'true' if args.omega != 1.0 else 'false'}>")
+        tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
-                    frac=achieved / PEAK_HBM, traffic=traffic_from_profiles(key),
+                    frac=achieved / PEAK_HBM, traffic=tr,
                     kernel=key, avg_launch_us=1e6 * avg_s, launches=nl,
-                    bytes_per_launch=per_launch_bytes)
+                    bytes_per_launch=per_launch_bytes, words_per_cell=words,
+                    cells_per_launch=cells / max(nl, 1))
         print(f"# smoother: {nl} launches, avg {1e6 * avg_s:.1f} us, {achieved / 1e9:.0f} GB/s, "
               f"tile widths fp64/fp32 {s.tile_widths().tolist()}", file=sys.stderr, flush=True)
 
@@ -232,7 +322,7 @@ def main():
             print("cpu baseline failed:", e, file=sys.stderr)
 
     out = {
-        "metric": "Mcell*V-cycles/s (time to 1e-10 residual reduction, mixed-prec vs FP64, 1 GPU)",
+        "metric": "Mcell\u00b7V-cycles/s + time to 1e-10 residual, mixed-prec vs FP64, 1 GPU",
         "value": value,
         "unit": "Mcell*V-cycles/s",
         "n_gpus": 1,
@@ -246,7 +336,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {p.nblk} blocks, {n1cells / 1e6:.1f}M cells, "
                                f"{s.nlev} levels, solve to {args.rtol:g} residual reduction",
-                   "mode": args.mode, "smoother": "red-black SOR" if args.mode != "exact" else
+                   "mode": args.mode, "smoother": "zebra y-line relaxation" if args.mode != "exact" else
                    "reference lexicographic SOR", "numpre": args.numpre, "numpost": args.numpost,
                    "sweeps_per_iter": args.spi, "omega": args.omega, "numcits": args.numcits},
         "time_to_rtol_ms": ms_step,

@@ -50,7 +50,7 @@ struct LevelDev {
     std::vector<LevBlk> h_lb;
     LevBlk *d_lb = nullptr;
     long tiles_int = 0, tiles_all = 0, tiles_rb = 0, tiles_ln = 0, nlines = 0;
-    long tiles_l2 = 0, tiles_l4 = 0;
+    long tiles_z64 = 0, tiles_z128 = 0, tiles_rr = 0, tiles_rp = 0;
     int smax = 0;                       // largest hyperplane index i+j+k (1-based)
     std::vector<Face> h_fc;
     Face *d_fc = nullptr;
@@ -187,7 +187,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
     for (int l = 0; l < L; ++l) {
         LevelDev &D = h->lev[l];
         D.h_lb.resize(h->nblk);
-        long ti = 0, ta = 0, tr = 0, tl = 0, tl2 = 0, tl4 = 0, sl = 0, ns = 0;
+        long ti = 0, ta = 0, tr = 0, tl = 0, tl2 = 0, tl4 = 0, trr = 0, trp = 0, sl = 0, ns = 0;
         double np = 0.0;
         long npi = 0;
         int smax = 0;
@@ -197,7 +197,10 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             memset(&B, 0, sizeof(B));
             B.off = h->off[l][b];
             B.n[0] = n[0]; B.n[1] = n[1]; B.n[2] = n[2];
-            B.hp = (n[0] + 1) / 2;
+            // row pitch of the split layout: ceil(nip1/2) slots rounded up to 32, so every row
+            // of every array starts on a 128-byte line (FP32) and a wavefront's 64 slots cover
+            // exactly two lines
+            B.hp = ((n[0] + 1) / 2 + 31) & ~31;
             B.off_s = ns;
             ns += (long)B.hp * n[1] * n[2];
             for (int s = 0; s < 6; ++s) {
@@ -212,10 +215,16 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             tr += (long)((B.hp + 63) / 64) * ((n[1] - 2 + 3) / 4);
             B.t_ln = tl;
             tl += (long)((B.hp + 63) / 64) * ((n[2] - 2 + 3) / 4);
-            B.t_l2 = tl2;
-            tl2 += (long)((B.hp + 127) / 128) * ((n[2] - 2 + 1) / 2);
-            B.t_l4 = tl4;
-            tl4 += (long)((B.hp + 255) / 256) * (n[2] - 2);
+            B.t_z64 = tl2;
+            tl2 += (long)((B.hp + 63) / 64) * ((n[2] - 2 + 1) / 2);
+            B.t_z128 = tl4;
+            tl4 += (long)((B.hp + 127) / 128) * (n[2] - 2);
+            B.t_rr = trr;
+            trr += (long)((((n[0] - 2) >> 1) + 63) / 64) * ((((n[2] - 2) >> 1) + 3) / 4) *
+                   ((((n[1] - 2) >> 1) + LMGX_RCH - 1) / LMGX_RCH);
+            B.t_rp = trp;
+            trp += (long)((((n[0] - 2) >> 1) + 63) / 64) * ((((n[2] - 2) >> 1) + 3) / 4) *
+                   ((((n[1] - 2) >> 1) + LMGX_RCHP - 1) / LMGX_RCHP);
             B.s_jk = sl;
             sl += (long)(n[1] - 2) * (n[2] - 2);
             B.o2 = h->off2d[l][b];
@@ -230,8 +239,10 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
         D.tiles_all = ta;
         D.tiles_rb = tr;
         D.tiles_ln = tl;
-        D.tiles_l2 = tl2;
-        D.tiles_l4 = tl4;
+        D.tiles_z64 = tl2;
+        D.tiles_z128 = tl4;
+        D.tiles_rr = trr;
+        D.tiles_rp = trp;
         D.nlines = sl;
         D.smax = smax;
         D.npts_int = npi;
@@ -305,6 +316,7 @@ static int upload_tables(lmgx_handle *h)
         swap_need = std::max(swap_need, D.nel);
         part_need = std::max(part_need, 4 * std::max(std::max(D.tiles_int, D.tiles_ln),
                                                      std::max(D.tiles_all, D.tiles_rb)));
+        part_need = std::max(part_need, 4 * D.tiles_rr);
     }
     HIPCHK(hipMalloc(&h->d_swap, sizeof(double) * swap_need));
     h->swap_cap = swap_need;

@@ -6,6 +6,8 @@
 #define LMGX_MAXB 64
 #define LMGX_MAXLEV 8
 #define LMGX_WG 256
+#define LMGX_RCH 1      // coarse rows per thread of the residual column kernel
+#define LMGX_RCHP 8     // coarse rows per thread of the prolongation column kernel
 
 #define BCT_PERIODIC 12
 #define BCT_INTERFACE 13
@@ -15,7 +17,7 @@ struct LevBlk {
     long off;        // mg_off: start of the block in the multi-level array
     long off_s;      // start of the block in each colour array of the split layout
     int n[3];        // nip1, njp1, nkp1 (halos included)
-    int hp;          // slots per (j,k) line per colour in the split layout = ceil(nip1/2)
+    int hp;          // row pitch of the split layout: ceil(nip1/2) slots rounded up to 32
     int ifc[6];      // side is an INTERFACE (correction range covers the halo)
     int swp[6];      // side is INTERFACE or PERIODIC (halo swapped)
     int dir[6];      // side carries a face-Dirichlet p = 0 (fast-path prolongation: odd extension)
@@ -23,8 +25,10 @@ struct LevBlk {
     long t_all;      // first tile of this block in the full (i,j) tiling (halos included)
     long t_rb;       // first tile of this block in the split-layout (slot,j) tiling
     long t_ln;       // first tile of this block in the split-layout (slot,k) line tiling 64x4
-    long t_l2;       // same, 128 slots x 2 k-planes per workgroup
-    long t_l4;       // same, 256 slots x 1 k-plane per workgroup
+    long t_z64;      // line-solve tiling: 64 slots x 2 k-planes, two wavefronts per line
+    long t_z128;     // line-solve tiling: 128 slots x 1 k-plane, two wavefronts per line
+    long t_rr;       // residual column tiling: 64 coarse i x 4 coarse k x LMGX_RCH coarse j
+    long t_rp;       // prolongation column tiling: 64 coarse i x 4 coarse k x LMGX_RCHP coarse j
     long s_jk;       // first (j,k) interior line of this block (hyperplane sweep)
     long o2;         // offset of the block's 2-D (i,j) metric arrays
     long o1;         // offset of the block's 1-D k metric arrays
@@ -58,6 +62,18 @@ __host__ __device__ inline int xcd_remap(int b, int G)
     return x * q + (x < r ? x : r) + i;
 }
 
+// Workgroup i runs on XCD i % 8.  Give each XCD runs of G consecutive tiles, dealt round-robin:
+// neighbouring k-planes of one block (which share the other colour's x) meet in one L2, and
+// every XCD still gets an even share of every block.  The tail that does not fill 8 runs keeps
+// the identity mapping.
+__device__ inline long xcd_chunk(long i, long n, int G)
+{
+    const long span = 8L * G;
+    if (i >= n / span * span) return i;
+    const long s = i >> 3;
+    return (s / G) * span + (i & 7) * G + (s % G);
+}
+
 // Find the block owning flattened tile t (start[] ascending, nblk <= 64).
 __device__ inline int find_block(const LevBlk *lb, int nblk, long t, int which)
 {
@@ -66,7 +82,8 @@ __device__ inline int find_block(const LevBlk *lb, int nblk, long t, int which)
         int mid = (lo + hi + 1) >> 1;
         long s = which == 0 ? lb[mid].t_int : which == 1 ? lb[mid].t_all : which == 2 ? lb[mid].t_rb
                  : which == 3 ? lb[mid].s_jk : which == 4 ? lb[mid].t_ln
-                 : which == 5 ? lb[mid].t_l2 : lb[mid].t_l4;
+                 : which == 5 ? lb[mid].t_z64 : which == 6 ? lb[mid].t_z128
+                 : which == 7 ? lb[mid].t_rr : lb[mid].t_rp;
         if (s <= t) lo = mid; else hi = mid - 1;
     }
     return lo;
