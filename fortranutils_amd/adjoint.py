@@ -1,0 +1,123 @@
+"""Differentiable pressure solve: p = A(rho)^-1 b behind ``torch.autograd.Function``.
+
+Forward: the FP64 multigrid solve (zebra line relaxation) of liblmgx on the operator built from
+the level-1 density.  Backward: the transposed-operator multigrid solve A^T lam = dJ/dp and the
+density adjoint kernel, both in liblmgx:
+    dJ/db = lam (interior cells),   dJ/dlog(rho) = -lam^T (dA/dlog rho) p.
+The pressure solve itself follows the reference's lmg_solve (code/lmg_solve.F); the adjoint has
+no counterpart in the reference.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import lmgx
+from . import problems as P
+
+ADJ_SOLVE = dict(smoother=1, precision=0, numpre=1, numpost=1, sweeps_per_iter=2, omega=1.0,
+                 omega_c=1.0, tolf=0.0, rtol=1e-13, max_cycles=80)
+
+
+class PressureOperator:
+    """One block topology with its geometry; the operator is rebuilt when the density changes."""
+
+    def __init__(self, problem: P.Problem, solve_kw=None):
+        import torch
+        self.problem = problem
+        self.solver = lmgx.Solver(lmgx.Topology.from_problem(problem))
+        self.dirichlet = np.array([b.dirichlet for b in problem.blocks], np.int32)
+        self.kw = dict(ADJ_SOLVE, **(solve_kw or {}))
+        self._rho_key = None
+        mask = np.concatenate([np.pad(np.ones([n - 2 for n in reversed(b.nijkp1)], bool), 1)
+                               .ravel() for b in problem.blocks])
+        self.mask = torch.from_numpy(mask).cuda()
+        self.last_cycles = 0
+
+    @property
+    def n1(self):
+        return self.solver.n1
+
+    def set_density(self, rho1):
+        """Build the operator from the level-1 density (device or host, n1 entries)."""
+        key = rho1.data_ptr(), rho1._version if hasattr(rho1, "_version") else None
+        r = rho1.detach().cpu().numpy() if hasattr(rho1, "detach") else np.asarray(rho1)
+        if self._rho_key is not None and self._rho_key[0] == key and \
+                np.array_equal(self._rho_key[1], r):
+            return
+        self.solver.build_operator(self.problem, rho1=r, dirichlet=self.dirichlet)
+        self._rho_key = (key, r.copy())
+
+    def _run(self, fn, rhs1):
+        import torch
+        s = self.solver
+        x = torch.zeros(s.ntot, dtype=torch.float64, device="cuda")
+        b = torch.zeros_like(x)
+        b[: s.n1] = rhs1
+        res = torch.zeros_like(x)
+        hist = fn(x, b, res, **self.kw)
+        self.last_cycles = len(hist)
+        return x[: s.n1].clone()
+
+    def solve(self, b1):
+        return self._run(self.solver.lmg_solve, b1)
+
+    def solve_adjoint(self, g1):
+        return self._run(self.solver.solve_adjoint, g1)
+
+
+class PressureSolve:
+    """``p = PressureSolve.apply(b1, logrho1, op)``: level-1 pressure (natural layout, n1)."""
+
+    _fn = None
+
+    @staticmethod
+    def apply(b1, logrho1, op: PressureOperator):
+        return _function().apply(b1, logrho1, op)
+
+
+def _function():
+    if PressureSolve._fn is None:
+        import torch
+
+        class _F(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, b1, logrho1, op):
+                rho = torch.exp(logrho1.detach())
+                op.set_density(rho)
+                p = op.solve(b1.detach())
+                ctx.save_for_backward(p, rho)
+                ctx.op = op
+                return p
+
+            @staticmethod
+            def backward(ctx, gp):
+                p, rho = ctx.saved_tensors
+                op = ctx.op
+                op.set_density(rho)
+                g = gp.detach().contiguous() * op.mask
+                lam = op.solve_adjoint(g)
+                gb = lam * op.mask if ctx.needs_input_grad[0] else None
+                glr = op.solver.rho_adjoint(lam, p) if ctx.needs_input_grad[1] else None
+                return gb, glr, None
+
+        PressureSolve._fn = _F
+    return PressureSolve._fn
+
+
+def probe_loss(p1, problem: P.Problem, box=16, corner=(40, 40, 24), block=0):
+    """Sum of p^2 over a box^3 probe in one block plus the mean pressure on the wall-adjacent
+    cells (first interior layer next to the two y walls) of every block."""
+    import torch
+    offs = np.cumsum([0] + [int(np.prod(b.nijkp1)) for b in problem.blocks])
+    nb = problem.blocks[block]
+    n0, n1_, n2 = nb.nijkp1
+    pb = p1[offs[block]: offs[block + 1]].view(n2, n1_, n0)
+    i0, j0, k0 = corner
+    probe = pb[1 + k0: 1 + k0 + box, 1 + j0: 1 + j0 + box, 1 + i0: 1 + i0 + box]
+    walls = []
+    for ib, b in enumerate(problem.blocks):
+        m0, m1, m2 = b.nijkp1
+        q = p1[offs[ib]: offs[ib + 1]].view(m2, m1, m0)
+        walls.append(q[1:-1, 1, 1:-1].reshape(-1))
+        walls.append(q[1:-1, m1 - 2, 1:-1].reshape(-1))
+    return (probe * probe).sum() + torch.cat(walls).mean()

@@ -75,6 +75,9 @@ struct lmgx_handle {
     LevelDev lev[LMGX_MAXLEV];
     hipStream_t stream = nullptr;
     double *d_a[7] = {nullptr};         // operator, natural layout, FP64
+    double *d_aT[7] = {nullptr};        // transposed operator (adjoint solves), built on demand
+    bool aT_valid = false;
+    double *d_rho_op = nullptr;         // densities the operator was built from (all levels)
     double *d_f = nullptr, *d_b = nullptr, *d_res = nullptr;  // host-API staging
     double *d_swap = nullptr;           // exchange buffer (FP64 elements)
     long swap_cap = 0;
@@ -355,7 +358,8 @@ extern "C" int lmgx_free(lmgx_handle *h)
         hipFree(h->lev[l].d_lb);
         hipFree(h->lev[l].d_fc);
     }
-    for (int q = 0; q < 7; ++q) hipFree(h->d_a[q]);
+    for (int q = 0; q < 7; ++q) { hipFree(h->d_a[q]); hipFree(h->d_aT[q]); }
+    hipFree(h->d_rho_op);
     hipFree(h->d_f); hipFree(h->d_b); hipFree(h->d_res);
     hipFree(h->d_swap); hipFree(h->d_partial); hipFree(h->d_scalar); hipFree(h->d_hist);
     hipFree(h->d_met); hipFree(h->d_dkp); hipFree(h->d_dkw);
@@ -452,6 +456,9 @@ extern "C" int lmgx_set_operator(lmgx_handle *h, const double *ap, const double 
                               h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_op = true;
+    h->aT_valid = false;
+    hipFree(h->d_rho_op);
+    h->d_rho_op = nullptr;
     return fast_operator_changed(h);
 }
 
@@ -464,6 +471,9 @@ extern "C" int lmgx_set_operator_device(lmgx_handle *h, const double *const d_co
                               hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_op = true;
+    h->aT_valid = false;
+    hipFree(h->d_rho_op);
+    h->d_rho_op = nullptr;
     return fast_operator_changed(h);
 }
 

@@ -198,13 +198,30 @@ class Solver:
                                            _dp(hist), C.byref(nc)))
         return hist[: nc.value]
 
-    def solve_adjoint(self, lam, g, res, *, max_cycles=50, **kw):
+    def solve_adjoint(self, lam, g, res, *, max_cycles=50, history=True, **kw):
+        """A^T lam = g with the transposed-operator multigrid (device tensors of ntot); same
+        keywords and history as :meth:`lmg_solve`."""
         p = default_params(max_cycles=max_cycles, **kw)
         hist = np.zeros((max(1, max_cycles), self.nlev, 4))
         nc = C.c_int()
         _check(lib().lmgx_solve_adjoint_device(self.h, _dp(lam), _dp(g), _dp(res), C.byref(p),
                                                _dp(hist), C.byref(nc)))
         return hist[: nc.value]
+
+    def transpose_operator(self):
+        """The transposed operator, 7 device tensors of ntot (ap aw ae as an al ar)."""
+        import torch
+        out = [torch.zeros(self.ntot, dtype=torch.float64, device="cuda") for _ in range(7)]
+        arr = (C.POINTER(C.c_double) * 7)(*[_dp(a) for a in out])
+        _check(lib().lmgx_transpose_operator_device(self.h, arr))
+        return out
+
+    def rho_adjoint(self, lam, p):
+        """dJ/dlog(rho) (device tensor of n1) from level-1 p (A p = b) and lam (A^T lam = dJ/dp)."""
+        import torch
+        g = torch.zeros(self.n1, dtype=torch.float64, device="cuda")
+        _check(lib().lmgx_rho_adjoint_device(self.h, _dp(lam), _dp(p), _dp(g)))
+        return g
 
     @property
     def jumble(self) -> int:

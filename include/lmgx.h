@@ -155,11 +155,21 @@ int lmgx_get_operator(const lmgx_handle *h, double *ap, double *aw, double *ae, 
 int lmgx_source_device(lmgx_handle *h, const double *d_u, const double *d_v, const double *d_w,
                        const double *d_rho, const double *d_rhonp1, double dtime, double *d_b);
 
-/* Adjoint: solve A^T lambda = g with the transposed-operator V-cycle (restriction = P^T,
- * prolongation = R^T, backward-ordered red-black smoothing), FP64, device arrays of ntot.
- * lambda: in = initial guess (level 1), out = solution.  Same history convention. */
+/* Adjoint (no counterpart in the reference; for gradients of pressure functionals).
+ *
+ * lmgx_solve_adjoint_device: A^T lambda = g with the same multigrid run on the explicitly
+ * transposed operator (row P's coupling to N = row N's coupling to P; across blocks, the
+ * partner side's coefficient).  Same parameters, arrays (device, ntot doubles) and history as
+ * lmgx_solve_device; lambda: in = initial guess on level 1, out = solution.
+ * lmgx_transpose_operator_device: the transposed operator (7 device arrays of ntot).
+ * lmgx_rho_adjoint_device: dJ/dlog(rho) for every level-1 entry of the density given to
+ * lmgx_build_operator, from the converged p (A p = b) and lambda (A^T lambda = dJ/dp); level-1
+ * device arrays (n1 doubles).  Entries the operator does not read get 0. */
 int lmgx_solve_adjoint_device(lmgx_handle *h, double *d_lambda, double *d_g, double *d_res,
                               const lmgx_params *prm, double *history, int *ncycles_out);
+int lmgx_transpose_operator_device(lmgx_handle *h, double *const d_out[7]);
+int lmgx_rho_adjoint_device(lmgx_handle *h, const double *d_lambda, const double *d_p,
+                            double *d_glogrho);
 
 /* Line-tile widths chosen by the fast path's autotuner: out[precision][LMGX_MAXL], precision 0 =
  * FP64, 1 = FP32; 0 for levels that do not exist or before the first fast solve. */

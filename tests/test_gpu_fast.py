@@ -51,3 +51,23 @@ def test_fast_converges_to_direct_solution(cfg, scale, prec):
     # the returned level-1 residual is the residual of the returned solution
     r = res.cpu().numpy()[: o.n1][m]
     assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(bh[m])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_new_operator_between_solves(prec):
+    """A second operator on the same handle (here 2A) must not reuse the first one's captured
+    cycle: the solution of 2A x = b is half that of A x = b."""
+    import torch
+    p, o, s, coeffs, b = setup("c4", 4)
+    kw = dict(max_cycles=60, tolf=0.0, rtol=1e-12, smoother=1, precision=prec, numpre=1,
+              numpost=1, omega=1.0, omega_c=1.0, sweeps_per_iter=1)
+    f1 = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    res = torch.zeros_like(f1)
+    s.lmg_solve(f1, b, res, **kw)
+    s.set_operator([2.0 * a for a in coeffs])
+    f2 = torch.zeros_like(f1)
+    s.lmg_solve(f2, b, res, **kw)
+    m = torch.from_numpy(interior_mask(o, 0)).cuda()
+    x1, x2 = f1[: o.n1][m], f2[: o.n1][m]
+    err = (torch.linalg.norm(2.0 * x2 - x1) / torch.linalg.norm(x1)).item()
+    assert err < 1e-9, err
