@@ -1,4 +1,4 @@
-"""Build liblmgx.so (HIP, gfx950) in-tree, and the Fortran drop-in shim when flang is present."""
+"""Build liblmgx.so (HIP, gfx950) in-tree, and the Fortran interface module when flang is present."""
 from __future__ import annotations
 
 import os
@@ -32,18 +32,17 @@ def build_lib(verbose: bool = False) -> str:
     return out
 
 
-def build_shim(verbose: bool = False) -> str | None:
-    """liblmg_shim.so: Fortran ``lmg_solve`` with the reference's 12-argument signature."""
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the Fortran interface module (fortran/lmgx_mod.f90) when flang is present; the
+    drop-in lmg_solve (fortran/lmg_solve_gpu.F) is compiled inside LESter's build, against
+    LESter's include files (oracle/Makefile, target ref-gpu, does exactly that)."""
     if not os.path.exists(FLANG):
         return None
-    src = os.path.join(HERE, "fortran", "lmg_solve_shim.f90")
-    if not os.path.exists(src):
-        return None
-    out = os.path.join(HERE, "liblmg_shim.so")
+    src = os.path.join(HERE, "fortran", "lmgx_mod.f90")
     moddir = os.path.join(HERE, "fortran", "_mod")
     os.makedirs(moddir, exist_ok=True)
-    cmd = [FLANG, "-O2", "-fPIC", "-shared", "-fdefault-real-8", "-module-dir", moddir,
-           "-o", out, src, "-L" + HERE, "-llmgx", "-Wl,-rpath,$ORIGIN"]
+    out = os.path.join(moddir, "lmgx_mod.o")
+    cmd = [FLANG, "-O2", "-fPIC", "-c", "-module-dir", moddir, "-o", out, src]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
@@ -54,7 +53,7 @@ def build_shim(verbose: bool = False) -> str | None:
 
 def build(verbose: bool = False) -> None:
     build_lib(verbose)
-    build_shim(verbose)
+    build_fortran(verbose)
 
 
 if __name__ == "__main__":

@@ -16,10 +16,12 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+# the same driver with the reference's lmg_solve replaced by the product's Fortran drop-in
+DRIVER_GPU = os.path.join(HERE, "_ref", "ref_driver_gpu")
 
 
-def available() -> bool:
-    return os.path.exists(DRIVER)
+def available(gpu: bool = False) -> bool:
+    return os.path.exists(DRIVER_GPU if gpu else DRIVER)
 
 
 def write_input(path: str, problem, *, ncycles: int, coeffs: Optional[Sequence[np.ndarray]] = None,
@@ -97,16 +99,17 @@ def read_output(path: str) -> Dict[str, np.ndarray]:
 
 def run(problem, *, ncycles: int, coeffs=None, b1=None, fields=None, f0=None, n1=None,
         do_post: int = 1, ntime: int = 1, tolf: float = 0.0, timeout: float = 3600.0,
-        cycle: int = 1):
-    if not available():
-        raise RuntimeError("reference driver not built: make -C oracle ref")
+        cycle: int = 1, gpu: bool = False, env_extra: Optional[dict] = None):
+    if not available(gpu):
+        raise RuntimeError("reference driver not built: make -C oracle ref ref-gpu")
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(fin, problem, ncycles=ncycles, coeffs=coeffs, b1=b1, fields=fields, f0=f0,
                     n1=n1, do_post=do_post, ntime=ntime, tolf=tolf, cycle=cycle)
         env = dict(os.environ)
-        r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, env=env,
-                           timeout=timeout)
+        env.update(env_extra or {})
+        r = subprocess.run([DRIVER_GPU if gpu else DRIVER, fin, fout], capture_output=True,
+                           text=True, env=env, timeout=timeout)
         if r.returncode != 0 or not os.path.exists(fout):
             raise RuntimeError(f"ref_driver failed ({r.returncode}): {r.stdout}\n{r.stderr}")
         return read_output(fout)
