@@ -147,3 +147,19 @@ def test_density_gradient_matches_cpu_pipeline():
         worst = max(worst, abs(fd - an) / abs(an))
     print("worst rel err vs CPU pipeline", worst)
     assert worst < 1e-6, worst
+
+
+@pytest.mark.parametrize("scale", [4])
+def test_dot_product_identity(scale):
+    """<A^-1 y, z> = <y, A^-T z> for the forward and the transposed-operator multigrid solves."""
+    import torch
+    p, o, op, b1, logrho, _, _ = make(scale)
+    op.set_density(torch.exp(logrho))
+    m = op.mask.to(torch.float64)
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    for _ in range(4):
+        y = torch.randn(op.n1, generator=gen, device="cuda", dtype=torch.float64) * m
+        z = torch.randn(op.n1, generator=gen, device="cuda", dtype=torch.float64) * m
+        lhs = float((op.solve(y) * z).sum())
+        rhs = float((y * op.solve_adjoint(z)).sum())
+        assert abs(lhs - rhs) <= 1e-10 * abs(lhs), (lhs, rhs)
