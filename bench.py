@@ -45,9 +45,11 @@ def parse():
     ap.add_argument("--scale", type=int, default=1)
     ap.add_argument("--mode", default="mixed", choices=["mixed", "fp64", "exact"])
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--numpre", type=int, default=1)
-    ap.add_argument("--numpost", type=int, default=1)
-    ap.add_argument("--spi", type=int, default=2, help="line-relaxation sweeps per smoothing iteration")
+    # V(2,2) with 3 line sweeps per smoothing iteration (the reference's smoothing iteration is 3
+    # SOR sweeps); the fastest schedule to 1e-10 measured on c3
+    ap.add_argument("--numpre", type=int, default=2)
+    ap.add_argument("--numpost", type=int, default=2)
+    ap.add_argument("--spi", type=int, default=3, help="line-relaxation sweeps per smoothing iteration")
     ap.add_argument("--numcits", type=int, default=20)
     ap.add_argument("--omega", type=float, default=1.0)
     ap.add_argument("--omega-c", type=float, default=1.0)
@@ -58,6 +60,7 @@ def parse():
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--pmc-summary", nargs="+", metavar="CSV")
+    ap.add_argument("--stats-summary", metavar="CSV", help="markdown table of a rocprofv3 kernel_stats.csv")
     ap.add_argument("--traffic", metavar="DIR")
     return ap.parse_args()
 
@@ -208,6 +211,22 @@ def pmc_summary(paths):
             print(f'"{k[:160]}",{n},{t:.1f},{t / n:.1f}')
 
 
+def stats_summary(path, top=25):
+    """The top kernels of a rocprofv3 --stats kernel_stats.csv as a markdown table."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    print("| kernel | calls | total ms | avg us | min us | max us | % |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for r in rows[:top]:
+        name = r["Name"].replace("void ", "").split("(")[0].replace("|", "/")
+        print(f"| `{name[:90]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+              f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+              f"{float(r['MaxNs']) / 1e3:.1f} | {100 * float(r['TotalDurationNs']) / tot:.1f} |")
+    print(f"\n{len(rows)} kernels, {tot / 1e6:.1f} ms in total")
+
+
 def traffic(d):
     """profiles/traffic.json entry from four rocprofv3 --pmc runs under directory ``d``:
     copy_FETCH_SIZE/, copy_WRITE_SIZE/ (``--copy-calib``: 3 copies of 1 GiB) and
@@ -257,6 +276,9 @@ def main():
         return
     if args.pmc_summary:
         pmc_summary(args.pmc_summary)
+        return
+    if args.stats_summary:
+        stats_summary(args.stats_summary)
         return
     if args.traffic:
         traffic(args.traffic)
