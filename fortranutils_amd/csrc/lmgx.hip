@@ -676,14 +676,14 @@ static int check_params(const lmgx_handle *h, const lmgx_params *p)
     if (p->cycle != 1 && p->cycle != 2) return fail(LMGX_E_ARG, "cycle must be 1 (V) or 2 (F)");
     if (p->numpre < 0 || p->numpost < 0 || p->numcits < 0 || p->max_cycles < 0)
         return fail(LMGX_E_ARG, "negative iteration count");
-    if (p->smoother != LMGX_SMOOTH_EXACT && p->smoother != LMGX_SMOOTH_RB)
+    if (p->smoother != LMGX_SMOOTH_EXACT && p->smoother != LMGX_SMOOTH_LINE)
         return fail(LMGX_E_ARG, "unknown smoother");
     if (p->precision != LMGX_PREC_FP64 && p->precision != LMGX_PREC_MIXED)
         return fail(LMGX_E_ARG, "unknown precision");
-    if (p->precision == LMGX_PREC_MIXED && p->smoother != LMGX_SMOOTH_RB)
-        return fail(LMGX_E_ARG, "mixed precision needs the red-black smoother");
-    if (p->smoother == LMGX_SMOOTH_RB && p->cycle != 1)
-        return fail(LMGX_E_UNSUPPORTED, "red-black path runs V-cycles only");
+    if (p->precision == LMGX_PREC_MIXED && p->smoother != LMGX_SMOOTH_LINE)
+        return fail(LMGX_E_ARG, "mixed precision needs the line smoother");
+    if (p->smoother == LMGX_SMOOTH_LINE && p->cycle != 1)
+        return fail(LMGX_E_UNSUPPORTED, "the line-smoother path runs V-cycles only");
     if (!h->have_op) return fail(LMGX_E_STATE, "lmgx_set_operator has not been called");
     return 0;
 }

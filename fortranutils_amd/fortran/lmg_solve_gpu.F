@@ -84,7 +84,7 @@ C=======================================================================
       prm%rtol       = 0.d0
       call get_environment_variable('LMGX_SMOOTHER', mode)
       if (mode(1:4) .eq. 'fast') then
-        prm%smoother  = LMGX_SMOOTH_RB
+        prm%smoother  = LMGX_SMOOTH_LINE
         prm%precision = LMGX_PREC_FP64
       else
         prm%smoother  = LMGX_SMOOTH_EXACT

@@ -6,9 +6,9 @@ module lmgx_mod
   private
   public :: lmgx_topology, lmgx_params, lmgx_init, lmgx_free, lmgx_levels, lmgx_default_params, &
             lmgx_set_operator, lmgx_solve, lmgx_last_error_str, &
-            LMGX_SMOOTH_EXACT, LMGX_SMOOTH_RB, LMGX_PREC_FP64, LMGX_PREC_MIXED
+            LMGX_SMOOTH_EXACT, LMGX_SMOOTH_LINE, LMGX_PREC_FP64, LMGX_PREC_MIXED
 
-  integer(c_int), parameter :: LMGX_SMOOTH_EXACT = 0, LMGX_SMOOTH_RB = 1
+  integer(c_int), parameter :: LMGX_SMOOTH_EXACT = 0, LMGX_SMOOTH_LINE = 1
   integer(c_int), parameter :: LMGX_PREC_FP64 = 0, LMGX_PREC_MIXED = 1
 
   type, bind(c) :: lmgx_topology

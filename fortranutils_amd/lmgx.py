@@ -22,7 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblmgx.so")
 
-LMGX_SMOOTH_EXACT, LMGX_SMOOTH_RB = 0, 1
+LMGX_SMOOTH_EXACT, LMGX_SMOOTH_LINE = 0, 1
 LMGX_PREC_FP64, LMGX_PREC_MIXED = 0, 1
 MAXL = 8
 

@@ -44,10 +44,10 @@ typedef struct lmgx_topology {
 
 /* smoother */
 #define LMGX_SMOOTH_EXACT 0   /* reference order: 3 lexicographic SOR sweeps per iteration */
-#define LMGX_SMOOTH_RB 1      /* zebra y-line relaxation (lines coloured by i+k), colour-split layout */
+#define LMGX_SMOOTH_LINE 1      /* zebra y-line relaxation (lines coloured by i+k), colour-split layout */
 /* precision */
 #define LMGX_PREC_FP64 0
-#define LMGX_PREC_MIXED 1     /* FP32 V-cycle inside FP64 iterative refinement (needs SMOOTH_RB) */
+#define LMGX_PREC_MIXED 1     /* FP32 V-cycle inside FP64 iterative refinement (needs SMOOTH_LINE) */
 
 typedef struct lmgx_params {
     int cycle;            /* 1 = V-cycle (meth_cycle), 2 = F-cycle (exact smoother only) */
@@ -59,7 +59,7 @@ typedef struct lmgx_params {
     double tolf;          /* reference stop test enorm/vnorm <= tolf (lmg_solve.F:82) */
     double rtol;          /* extra stop test rnorm_k/rnorm_0 <= rtol, 0 = off */
     int smoother;         /* LMGX_SMOOTH_* */
-    int precision;        /* LMGX_PREC_* (MIXED needs LMGX_SMOOTH_RB) */
+    int precision;        /* LMGX_PREC_* (MIXED needs LMGX_SMOOTH_LINE) */
     int check_every;      /* fast path: test convergence every N cycles (>= 1) */
     int sweeps_per_iter;  /* fast path: zebra line sweeps per smoothing iteration (reference SOR: 3) */
     int coarse_op;        /* fast path: 0 = coarse levels as given (re-discretised, as coeff_p),
