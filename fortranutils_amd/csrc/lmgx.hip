@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -50,7 +51,8 @@ struct LevelDev {
     std::vector<LevBlk> h_lb;
     LevBlk *d_lb = nullptr;
     long tiles_int = 0, tiles_all = 0, tiles_rb = 0, tiles_ln = 0, nlines = 0;
-    long tiles_z64 = 0, tiles_z128 = 0, tiles_rr = 0, tiles_rp = 0;
+    long tiles_z64 = 0, tiles_z128 = 0, tiles_rr = 0, tiles_rp = 0, tiles_sg = 0;
+    int njmax = 0;                      // longest j-line (interior rows) of the level
     int smax = 0;                       // largest hyperplane index i+j+k (1-based)
     std::vector<Face> h_fc;
     Face *d_fc = nullptr;
@@ -190,7 +192,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
     for (int l = 0; l < L; ++l) {
         LevelDev &D = h->lev[l];
         D.h_lb.resize(h->nblk);
-        long ti = 0, ta = 0, tr = 0, tl = 0, tl2 = 0, tl4 = 0, trr = 0, trp = 0, sl = 0, ns = 0;
+        long ti = 0, ta = 0, tr = 0, tl = 0, tl2 = 0, tl4 = 0, trr = 0, trp = 0, tsg = 0, sl = 0, ns = 0;
         double np = 0.0;
         long npi = 0;
         int smax = 0;
@@ -225,6 +227,17 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             B.t_rr = trr;
             trr += (long)((((n[0] - 2) >> 1) + 63) / 64) * ((((n[2] - 2) >> 1) + 3) / 4) *
                    ((((n[1] - 2) >> 1) + LMGX_RCH - 1) / LMGX_RCH);
+            {
+                // segmented line solve: fewest power-of-two segments of <= LMGX_SMAX rows
+                const int nj = n[1] - 2;
+                int ns = 1;
+                while (ns < 16 && (nj + ns - 1) / ns > LMGX_SMAX) ns *= 2;
+                B.sg_n = ns;
+                B.sg_r = (nj + ns - 1) / ns;
+                B.sg_s = 16 / ns;
+            }
+            B.t_sg = tsg;
+            tsg += (long)((B.hp + 63) / 64) * ((n[2] - 2 + B.sg_s - 1) / B.sg_s);
             B.t_rp = trp;
             trp += (long)((((n[0] - 2) >> 1) + 63) / 64) * ((((n[2] - 2) >> 1) + 3) / 4) *
                    ((((n[1] - 2) >> 1) + LMGX_RCHP - 1) / LMGX_RCHP);
@@ -237,6 +250,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             np = np + (double)((long)(n[0] - 2) * (n[1] - 2) * (n[2] - 2));
             npi += (long)(n[0] - 2) * (n[1] - 2) * (n[2] - 2);
             smax = std::max(smax, n[0] - 1 + n[1] - 1 + n[2] - 1);
+            D.njmax = std::max(D.njmax, n[1] - 2);
         }
         D.tiles_int = ti;
         D.tiles_all = ta;
@@ -246,6 +260,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
         D.tiles_z128 = tl4;
         D.tiles_rr = trr;
         D.tiles_rp = trp;
+        D.tiles_sg = tsg;
         D.nlines = sl;
         D.smax = smax;
         D.npts_int = npi;

@@ -8,6 +8,8 @@
 #define LMGX_WG 256
 #define LMGX_RCH 1      // coarse rows per thread of the residual column kernel
 #define LMGX_RCHP 8     // coarse rows per thread of the prolongation column kernel
+#define LMGX_SMAX 16    // segmented line solve: rows per segment held in registers, 16 segments
+                        // at most, so lines up to 256 rows
 
 #define BCT_PERIODIC 12
 #define BCT_INTERFACE 13
@@ -29,6 +31,10 @@ struct LevBlk {
     long t_z128;     // line-solve tiling: 128 slots x 1 k-plane, two wavefronts per line
     long t_rr;       // residual column tiling: 64 coarse i x 4 coarse k x LMGX_RCH coarse j
     long t_rp;       // prolongation column tiling: 64 coarse i x 4 coarse k x LMGX_RCHP coarse j
+    long t_sg;       // segmented line-solve tiling: 64 slots x sg_s k-planes per 16 wavefronts
+    int sg_n;        // segmented line solve: segments per line (power of two <= 16)
+    int sg_r;        // rows per segment (<= LMGX_SMAX)
+    int sg_s;        // line sets (k-planes) per workgroup = 16 / sg_n
     long s_jk;       // first (j,k) interior line of this block (hyperplane sweep)
     long o2;         // offset of the block's 2-D (i,j) metric arrays
     long o1;         // offset of the block's 1-D k metric arrays
@@ -83,7 +89,7 @@ __device__ inline int find_block(const LevBlk *lb, int nblk, long t, int which)
         long s = which == 0 ? lb[mid].t_int : which == 1 ? lb[mid].t_all : which == 2 ? lb[mid].t_rb
                  : which == 3 ? lb[mid].s_jk : which == 4 ? lb[mid].t_ln
                  : which == 5 ? lb[mid].t_z64 : which == 6 ? lb[mid].t_z128
-                 : which == 7 ? lb[mid].t_rr : lb[mid].t_rp;
+                 : which == 7 ? lb[mid].t_rr : which == 8 ? lb[mid].t_rp : lb[mid].t_sg;
         if (s <= t) lo = mid; else hi = mid - 1;
     }
     return lo;

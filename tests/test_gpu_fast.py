@@ -71,3 +71,30 @@ def test_new_operator_between_solves(prec):
     x1, x2 = f1[: o.n1][m], f2[: o.n1][m]
     err = (torch.linalg.norm(2.0 * x2 - x1) / torch.linalg.norm(x1)).item()
     assert err < 1e-9, err
+
+
+@pytest.mark.parametrize("kernel", ["64", "128", "seg"])
+@pytest.mark.parametrize("cfg,scale", [("c1", 4), ("c4", 4)])
+def test_line_kernels_converge(kernel, cfg, scale, monkeypatch):
+    """Every line kernel (two-ended with 64 or 128 lines per wavefront row; segmented, FP32),
+    pinned on every level where it applies, over-relaxed, mixed precision."""
+    import torch
+    import scipy.sparse.linalg as spla
+    monkeypatch.setenv("LMGX_LINE_KERNEL", kernel)
+    p, o, s, coeffs, b = setup(cfg, scale)
+    A, idx = assemble_level1(o, coeffs)
+    bh = b.cpu().numpy()[: o.n1]
+    m = interior_mask(o, 0)
+    x_ref = spla.spsolve(A.tocsc(), bh[m])
+    f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    res = torch.zeros_like(f)
+    hist = s.lmg_solve(f, b, res, max_cycles=60, tolf=0.0, rtol=1e-11, smoother=1, precision=1,
+                       numpre=1, numpost=1, omega=1.3, omega_c=1.0, sweeps_per_iter=2)
+    tw = s.tile_widths()
+    print(cfg, kernel, "cycles", len(hist), "widths", tw.tolist())
+    if kernel == "seg":
+        assert (tw[1] == 16).all()
+    assert len(hist) < 60
+    x = f.cpu().numpy()[: o.n1][m]
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert err < 1e-9, err
