@@ -256,7 +256,8 @@ def traffic(d):
                                                   recursive=True)[0]))
         # every finest-level line-kernel instance (tile width, relaxation) separately
         names = sorted({n.replace("void ", "").split("(")[0] for n, _ in fetch_rows
-                        if n.replace("void ", "").startswith(f"k_zebra_line<{word}, 1,")})
+                        if n.replace("void ", "").startswith((f"k_zebra_line<{word}, 1,",
+                                                              f"k_zebra_seg<{word}, 1,"))})
         ent = {}
         for kname in names:
             pick = lambda name, k=kname: name.replace("void ", "").split("(")[0] == k
@@ -313,20 +314,22 @@ def main():
               f" |p_mixed - p_fp64|/|p_fp64| = {err:.2e}", file=sys.stderr, flush=True)
 
     # dominant kernel: the finest-level line-relaxation half-sweep; HIP events on the solver's
-    # stream.  Algorithmic bytes per updated cell: 8 operator words (b, aw, ae, al, ar, as|an,
-    # inv, cp), 1 word of the other colour's x, 1 word written (x) - 10 words; 11 when
-    # over-relaxed (the old x is read).
+    # stream.  Algorithmic bytes per updated cell, whatever the kernel: the right-hand side b,
+    # 6 operator words (ap, aw, ae, al, ar and the line coupling as; an is the next row's as),
+    # 1 word of the other colour's x, 1 word written (x) - 9 words; 10 when over-relaxed (the
+    # old x is read).
     roof = None
     if args.mode != "exact":
         _, _, (ms, nl, cells) = solve(s, b, args, args.mode, args.rtol, profile=True)
         word = 4 if args.mode == "mixed" else 8
-        words = 10 if args.omega == 1.0 else 11
+        words = 9 if args.omega == 1.0 else 10
         per_launch_bytes = (cells / max(nl, 1)) * words * word
         avg_s = ms / max(nl, 1) / 1e3
         achieved = per_launch_bytes / avg_s
         tm = int(s.tile_widths()[1 if word == 4 else 0][0])
-        key = (f"k_zebra_line<{'float' if word == 4 else 'double'}, 1, {tm}, "
-               f"{'true' if args.omega != 1.0 else 'false'}>")
+        rl = "true" if args.omega != 1.0 else "false"
+        fp = "float" if word == 4 else "double"
+        key = (f"k_zebra_seg<{fp}, 1, {rl}>" if tm == 16 else f"k_zebra_line<{fp}, 1, {tm}, {rl}>")
         tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=achieved / PEAK_HBM, traffic=tr,
