@@ -301,7 +301,7 @@ def main():
     value = n1cells * cycles / dt / 1e6
     red = hist[-1, 0, 0] / hist[0, 0, 0] if len(hist) else float("nan")
     print(f"# {args.mode}: {cycles / args.steps:.1f} cycles/solve, {ms_step:.1f} ms/solve, "
-          f"final reduction {red:.2e}", file=sys.stderr, flush=True)
+          f"rnorm last / first cycle {red:.2e}", file=sys.stderr, flush=True)
 
     extra = {}
     if not args.no_compare and args.mode != "fp64":
