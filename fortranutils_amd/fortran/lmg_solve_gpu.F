@@ -33,44 +33,15 @@ C=======================================================================
       real      ap(*), aw(*), ae(*), as(*), an(*), al(*), ar(*)
       real      f(*), b(*), residu(*)
 
-      type(c_ptr), save :: h = c_null_ptr
-      integer(c_int), save :: nlev = 0
-      integer(c_int), allocatable, target, save :: tdims(:,:)
-      integer(c_int), allocatable, target, save :: tbc(:,:)
-      integer(c_int), allocatable, target, save :: tbl(:,:)
-      integer(c_int), allocatable, target, save :: tsd(:,:)
-      type(lmgx_topology) :: topo
+      type(c_ptr) :: h
+      integer(c_int) :: nlev
       type(lmgx_params) :: prm
       real(c_double), allocatable :: hist(:,:,:)
       integer(c_int) :: rc, ncyc
-      integer(c_long) :: ntot, n1
-      integer(c_int), allocatable :: dims(:)
-      integer(c_long), allocatable :: off(:)
       character(len=16) :: mode
       integer :: l
 
-      if (.not. c_associated(h)) then
-        allocate(tdims(3,numbl), tbc(6,numbl), tbl(6,numbl),
-     &           tsd(6,numbl))
-        tdims = bl_nijkp1(:,1:numbl)
-        tbc   = bl_bc(:,1:numbl)
-        tbl   = bl2bl(:,1:numbl)
-        tsd   = bl2side(:,1:numbl)
-        topo%nblk    = numbl
-        topo%nlev    = nummgl
-        topo%nijkp1  = c_loc(tdims)
-        topo%bc      = c_loc(tbc)
-        topo%bl2bl   = c_loc(tbl)
-        topo%bl2side = c_loc(tsd)
-        rc = lmgx_init(topo, h)
-        if (rc .ne. 0) then
-          write(*,*) 'lmg_solve (liblmgx): ', lmgx_last_error_str()
-          stop 'lmg_solve: liblmgx init failed'
-        end if
-        allocate(dims(3*maxbl*8), off(maxbl*8))
-        rc = lmgx_levels(h, nlev, ntot, n1, dims, off)
-        deallocate(dims, off)
-      end if
+      call lmgx_attach(h, nlev)
 
       call lmgx_default_params(prm)
       prm%cycle      = meth_cycle
@@ -108,6 +79,66 @@ C=======================================================================
         write(*,*) 'lmg_solve (liblmgx): ', lmgx_last_error_str()
         stop 'lmg_solve: liblmgx failed'
       end if
+
+      return
+      end
+
+
+C=======================================================================
+C  The run's liblmgx handle (module variable lmgx_shared), created on
+C  the first call from LESter's topology commons; nlev = the level count
+C  the library settled on.  Used by the drop-ins lmg_solve and uvw_adv.
+C=======================================================================
+
+      subroutine lmgx_attach(h, nlev)
+
+      use iso_c_binding
+      use lmgx_mod
+
+      implicit none
+
+#include "dim.inc"
+#include "cb.lgrid.inc"
+#include "cb.control.inc"
+
+      type(c_ptr), intent(out) :: h
+      integer(c_int), intent(out) :: nlev
+
+      integer(c_int), save :: nlev_s = 0
+      integer(c_int), allocatable, target, save :: tdims(:,:)
+      integer(c_int), allocatable, target, save :: tbc(:,:)
+      integer(c_int), allocatable, target, save :: tbl(:,:)
+      integer(c_int), allocatable, target, save :: tsd(:,:)
+      type(lmgx_topology) :: topo
+      integer(c_int) :: rc
+      integer(c_long) :: ntot, n1
+      integer(c_int), allocatable :: dims(:)
+      integer(c_long), allocatable :: off(:)
+
+      if (.not. c_associated(lmgx_shared)) then
+        allocate(tdims(3,numbl), tbc(6,numbl), tbl(6,numbl),
+     &           tsd(6,numbl))
+        tdims = bl_nijkp1(:,1:numbl)
+        tbc   = bl_bc(:,1:numbl)
+        tbl   = bl2bl(:,1:numbl)
+        tsd   = bl2side(:,1:numbl)
+        topo%nblk    = numbl
+        topo%nlev    = nummgl
+        topo%nijkp1  = c_loc(tdims)
+        topo%bc      = c_loc(tbc)
+        topo%bl2bl   = c_loc(tbl)
+        topo%bl2side = c_loc(tsd)
+        rc = lmgx_init(topo, lmgx_shared)
+        if (rc .ne. 0) then
+          write(*,*) 'liblmgx: ', lmgx_last_error_str()
+          stop 'liblmgx init failed'
+        end if
+        allocate(dims(3*maxbl*8), off(maxbl*8))
+        rc = lmgx_levels(lmgx_shared, nlev_s, ntot, n1, dims, off)
+        deallocate(dims, off)
+      end if
+      h = lmgx_shared
+      nlev = nlev_s
 
       return
       end

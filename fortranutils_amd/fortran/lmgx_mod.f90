@@ -5,8 +5,12 @@ module lmgx_mod
   implicit none
   private
   public :: lmgx_topology, lmgx_params, lmgx_init, lmgx_free, lmgx_levels, lmgx_default_params, &
-            lmgx_set_operator, lmgx_solve, lmgx_last_error_str, &
+            lmgx_set_operator, lmgx_solve, lmgx_uvw_adv, lmgx_last_error_str, lmgx_shared, &
             LMGX_SMOOTH_EXACT, LMGX_SMOOTH_LINE, LMGX_PREC_FP64, LMGX_PREC_MIXED
+
+  ! the one handle of a LESter run, created on first use by lmgx_attach (lmg_solve_gpu.F) and
+  ! shared by the drop-ins lmg_solve and uvw_adv
+  type(c_ptr), save :: lmgx_shared = c_null_ptr
 
   integer(c_int), parameter :: LMGX_SMOOTH_EXACT = 0, LMGX_SMOOTH_LINE = 1
   integer(c_int), parameter :: LMGX_PREC_FP64 = 0, LMGX_PREC_MIXED = 1
@@ -62,6 +66,16 @@ module lmgx_mod
       type(lmgx_params), intent(in) :: p
       real(c_double), intent(out) :: hist(*)
       integer(c_int), intent(out) :: ncyc
+    end function
+
+    integer(c_int) function lmgx_uvw_adv(h, ivar, diu, rp, dkw, u, u_int, rho, p, dtime) &
+        bind(c, name='lmgx_uvw_adv')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: ivar
+      real(c_double), intent(in) :: diu(*), rp(*), dkw(*), u_int(*), rho(*), p(*)
+      real(c_double), intent(inout) :: u(*)
+      real(c_double), value :: dtime
     end function
 
     type(c_ptr) function lmgx_last_error() bind(c, name='lmgx_last_error')

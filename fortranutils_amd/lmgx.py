@@ -74,12 +74,20 @@ def _check(rc: int):
 
 
 def _dp(a) -> C.POINTER(C.c_double):
-    """Pointer to a float64 numpy array or a torch tensor (host or device)."""
+    """Pointer to a float64 numpy array or a torch tensor (host or device).
+
+    The library works on its own HIP stream, so a device tensor is handed over only after the
+    work torch has queued on its current stream (which may still be writing the tensor) is done.
+    The library synchronises its own stream before it returns, so its results are complete for
+    torch."""
     if isinstance(a, np.ndarray):
         assert a.dtype == np.float64 and a.flags.c_contiguous
         return a.ctypes.data_as(C.POINTER(C.c_double))
     # torch tensor
     assert a.is_contiguous(), "tensor must be contiguous"
+    if a.is_cuda:
+        import torch
+        torch.cuda.current_stream(a.device).synchronize()
     return C.cast(C.c_void_p(a.data_ptr()), C.POINTER(C.c_double))
 
 
@@ -181,6 +189,23 @@ class Solver:
     def source(self, u, v, w, rho, rhonp1, dtime, b):
         _check(lib().lmgx_source_device(self.h, _dp(u), _dp(v), _dp(w), _dp(rho), _dp(rhonp1),
                                         C.c_double(dtime), _dp(b)))
+
+    def uvw_adv(self, ivar, u, u_int, rho, p, dtime, diu=None, rp=None, dkw=None):
+        """Velocity correction after the solve (reference ``code/uvw_adv.F:5``): component ``ivar``
+        (1 U, 2 V, 3 W) of ``u`` from ``u_int``, ``rho`` (rho^{n+1}) and the pressure ``p``, then
+        the interface/periodic swap of ``u``.  numpy arrays -> host entry point with the
+        caller's metrics ``diu`` (djv for V), ``rp``, ``dkw``; torch CUDA tensors -> device entry
+        point, metrics optional (default: those of ``build_operator``)."""
+        if isinstance(u, np.ndarray):
+            if diu is None or rp is None or dkw is None:
+                raise LmgxError("host uvw_adv needs the metrics diu (djv for V), rp and dkw")
+            _check(lib().lmgx_uvw_adv(self.h, int(ivar), _dp(diu), _dp(rp), _dp(dkw), _dp(u),
+                                      _dp(u_int), _dp(rho), _dp(p), C.c_double(dtime)))
+        else:
+            m = [None if a is None else _dp(a) for a in (diu, rp, dkw)]
+            _check(lib().lmgx_uvw_adv_device(self.h, int(ivar), *m, _dp(u), _dp(u_int), _dp(rho),
+                                             _dp(p), C.c_double(dtime)))
+        return u
 
     # ---- solve --------------------------------------------------------------------------
     def lmg_solve(self, f, b, residu, *, max_cycles=5, history=True, **kw):

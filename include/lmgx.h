@@ -10,6 +10,11 @@
  *
  * Threading: one host thread per handle at a time.  A handle owns its device memory and one
  * HIP stream; host pointers are borrowed for the duration of a call only.
+ *
+ * Device arrays (the *_device entry points) are read and written on the handle's own stream,
+ * which is synchronised before the call returns.  Work the caller has queued on other streams
+ * is not waited for: the caller makes its inputs ready first (synchronises the stream that
+ * produced them; the Python binding does this for torch's current stream).
  */
 #ifndef LMGX_H
 #define LMGX_H
@@ -154,6 +159,24 @@ int lmgx_get_operator(const lmgx_handle *h, double *ap, double *aw, double *ae, 
  * requires a prior lmgx_build_operator (metrics). */
 int lmgx_source_device(lmgx_handle *h, const double *d_u, const double *d_v, const double *d_w,
                        const double *d_rho, const double *d_rhonp1, double dtime, double *d_b);
+
+/* Velocity correction after the pressure solve; replaces reference code/uvw_adv.F:5
+ * uvw_adv(ivar1, diu, rp, dkw, u, u_int, rho, p, wk, offwork, step) with its block routine
+ * code/uvw_adv_block.F:5 and the swap_sc of u at code/uvw_adv.F:59.  For ivar = 1 (U), 2 (V) or
+ * 3 (W): u = u_int - 1.5 dtime (p_P - p_N) / d / (0.5 (rho_P + rho_N)) on the component's level-1
+ * range, then the interface/periodic halos of u are swapped.  As at code/step.F:418-425, diu is
+ * the U/W metric and the caller passes djv in its place for V; rho is rho^{n+1}.  u, u_int, rho:
+ * level-1 layout (n1 doubles); p: level-1 part of the multi-level pressure; diu, rp: the
+ * multi-level 2-D arrays (level-1 part read), dkw: the multi-level 1-D k array.  u keeps its
+ * values outside the corrected range.
+ * lmgx_uvw_adv: host arrays (copied over PCIe each call).
+ * lmgx_uvw_adv_device: device arrays; d_diu, d_rp, d_dkw may all be NULL to use the metrics of
+ * lmgx_build_operator (diu, or djv for V). */
+int lmgx_uvw_adv(lmgx_handle *h, int ivar, const double *diu, const double *rp, const double *dkw,
+                 double *u, const double *u_int, const double *rho, const double *p, double dtime);
+int lmgx_uvw_adv_device(lmgx_handle *h, int ivar, const double *d_diu, const double *d_rp,
+                        const double *d_dkw, double *d_u, const double *d_u_int,
+                        const double *d_rho, const double *d_p, double dtime);
 
 /* Adjoint (no counterpart in the reference; for gradients of pressure functionals).
  *

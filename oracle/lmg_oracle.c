@@ -987,4 +987,58 @@ void lmgo_sour(const lmgo *g, const double *met, const double *dkp_, const doubl
     }
 }
 
+/* ------------------------------------------------------------------------------------------
+ * Velocity correction, code/uvw_adv.F:36-62 + code/uvw_adv_block.F:39-102 (level 1):
+ *   u = u_int - 1.5 dt (p_P - p_N) / d / (0.5 (rho_P + rho_N)),  N the lower neighbour along the
+ * component's axis, d = diu (U), djv (V) or rp*dkw(k) (W) - step.F:418-425 passes djv as the
+ * metric of the V call.  Range per component from code/set_loop_bound.F:64-82 (the first face is
+ * skipped on a non-periodic, non-interface lower side); the W loop over i starts at jstr as in
+ * uvw_adv_block.F:88 (jstr = istr = 2 for W).  Then swap_sc of u (uvw_adv.F:59).
+ * ------------------------------------------------------------------------------------------ */
+void lmgo_uvw_adv(const lmgo *g, int ivar, const double *met, const double *dkw_, double *u_,
+                  const double *uint_, const double *rho_, const double *p_, double dtime)
+{
+    const long n2d = g->n2d;
+    const double ddtime = 1.5 * dtime;
+    for (int bl = 0; bl < g->nblk; ++bl) {
+        const int *n = g->nijkp1[0][bl];
+        const int nip1 = n[0], njp1 = n[1], ni = n[0] - 1, nj = n[1] - 1, nk = n[2] - 1;
+        const int *bc = g->bc[bl];
+        const int istr = (ivar == 1 && !is_swap_bc(bc[0])) ? 3 : 2;
+        const int jstr = (ivar == 2 && !is_swap_bc(bc[2])) ? 3 : 2;
+        const int kstr = (ivar == 3 && !is_swap_bc(bc[4])) ? 3 : 2;
+        const long jo = nip1, ko = (long)nip1 * njp1;
+        const long o2 = g->off2d[0][bl], o = g->off[0][bl];
+        const double *d = met + (ivar == 2 ? 1 : 0) * n2d + o2;   /* diu, or djv for V */
+        const double *rp = met + 6 * n2d + o2, *dkw = dkw_ + g->off1d[0][bl];
+        double *u = u_ + o;
+        const double *ui = uint_ + o, *rho = rho_ + o, *p = p_ + o;
+        if (ivar != 3) {
+            const long io_or_jo = ivar == 1 ? 1 : jo;
+            for (int k = kstr; k <= nk; ++k)
+                for (int j = jstr; j <= nj; ++j)
+                    for (int i = istr; i <= ni; ++i) {
+                        const long ijk = (i - 1) + jo * (j - 1) + ko * (k - 1);
+                        const long ijkw = ijk - io_or_jo;
+                        const long ij = (i - 1) + (long)nip1 * (j - 1);
+                        const double dpdi = (p[ijk] - p[ijkw]) / d[ij];
+                        const double rhou = 0.5 * (rho[ijk] + rho[ijkw]);
+                        u[ijk] = ui[ijk] - ddtime * dpdi / rhou;
+                    }
+        } else {
+            for (int k = kstr; k <= nk; ++k)
+                for (int j = jstr; j <= nj; ++j)
+                    for (int i = jstr; i <= ni; ++i) {
+                        const long ijk = (i - 1) + jo * (j - 1) + ko * (k - 1);
+                        const long ijkl = ijk - ko;
+                        const long ij = (i - 1) + (long)nip1 * (j - 1);
+                        const double dpdi = (p[ijk] - p[ijkl]) / (rp[ij] * dkw[k - 1]);
+                        const double rhou = 0.5 * (rho[ijk] + rho[ijkl]);
+                        u[ijk] = ui[ijk] - ddtime * dpdi / rhou;
+                    }
+        }
+    }
+    lmgo_swap(g, 0, u_);
+}
+
 unsigned long lmgo_sizeof(void) { return (unsigned long)sizeof(lmgo); }

@@ -80,6 +80,11 @@ void lmgo_sour(const lmgo *g, const double *met, const double *dkp, const double
                const double *u, const double *v, const double *w, const double *rho,
                const double *rhonp1, double dtime, double *b);
 
+/* uvw_adv on level 1 for ivar = 1 (U), 2 (V), 3 (W): corrects u from u_int over the component's
+ * loop range, then swaps u.  met as from lmgo_geometry (diu, djv and rp are read). */
+void lmgo_uvw_adv(const lmgo *g, int ivar, const double *met, const double *dkw, double *u,
+                  const double *u_int, const double *rho, const double *p, double dtime);
+
 #ifdef __cplusplus
 }
 #endif

@@ -124,6 +124,14 @@ class Oracle:
                         _p(w), _p(rho), _p(rhonp1), C.c_double(dtime), _p(b))
         return b
 
+    def uvw_adv(self, ivar, geo, u, u_int, rho, p, dtime):
+        """Velocity correction of component ivar (1 U, 2 V, 3 W) into u (level-1 layout), then the
+        level-1 swap of u.  p may be the multi-level array (its level-1 part is read)."""
+        met = np.ascontiguousarray(geo["met"]).ravel()
+        lib().lmgo_uvw_adv(C.byref(self.g), int(ivar), _p(met), _p(geo["dkw"]), _p(u),
+                           _p(u_int), _p(rho), _p(p), C.c_double(dtime))
+        return u
+
     def solve(self, coeffs, f, b, ncycles, cycle=1, tolf=0.0, res=None):
         p = self.problem
         prm = Params(cycle, p.numpre, p.numpost, p.numcits, ncycles, p.omega, p.omega_c, tolf)

@@ -9,7 +9,7 @@ C step.F:114-129,222,353-380:
 C   set_offset -> set_const -> set_loop_bound(SV_P) -> set_interfaces
 C   -> mg_xyz(x,y,z) -> comp_metric -> coeff_p -> sour_p
 C   -> lmg_solve (one call per V-cycle, norms logged after each)
-C   -> extrap_p -> p_ref_subtract
+C   -> extrap_p -> p_ref_subtract -> uvw_adv (U, V, W)
 C The commons are filled directly instead of through input.dat /
 C block.dat / x.bin, since only their values matter to the path.
 C
@@ -51,7 +51,8 @@ C=======================================================================
       real, allocatable :: ap(:), aw(:), ae(:), as(:), an(:), al(:),
      &     ar(:), f(:), b(:), res(:), f0(:)
       real, allocatable :: u(:), v(:), w(:), rho(:), rhonp1(:),
-     &     drho(:), srhot(:), srhoux(:), gia(:), gja(:)
+     &     drho(:), srhot(:), srhoux(:), gia(:), gja(:),
+     &     uo(:), vo(:), wo(:)
       real, allocatable :: x(:), y(:), z(:), work(:), geo(:)
       real, allocatable :: m2(:,:), dkp(:), dkw(:)
       integer, allocatable :: pnti(:,:)
@@ -256,6 +257,28 @@ C POST-SOLVE: ZERO-GRADIENT HALOS + SWAP, THEN REFERENCE-POINT SHIFT
         call p_ref_subtract(f)
       end if
       write(11) (f(i), i=1,n1)
+
+C VELOCITY CORRECTION WITH THE FINAL p (step.F:418-425), WHEN THE
+C VELOCITIES WERE GIVEN: u_int = THE INPUT u, v, w; rho = rhonp1;
+C THE CORRECTED FIELDS START AS COPIES OF u_int.  METRICS diu=2,
+C djv=10, rp=27 (V PASSES djv IN PLACE OF diu, AS step.F:421 DOES)
+
+      if (src_src .eq. 0 .and. do_post .ne. 0) then
+        allocate(uo(n1), vo(n1), wo(n1))
+        uo = u
+        vo = v
+        wo = w
+        call uvw_adv(SV_U,m2(1,2),m2(1,27),dkw,uo,u,rhonp1,f,
+     &       work,iwork,1)
+        call uvw_adv(SV_V,m2(1,10),m2(1,27),dkw,vo,v,rhonp1,f,
+     &       work,iwork,1)
+        call uvw_adv(SV_W,m2(1,2),m2(1,27),dkw,wo,w,rhonp1,f,
+     &       work,iwork,1)
+        write(11) n1
+        write(11) uo, vo, wo
+      else
+        write(11) 0
+      end if
       close(11)
 
       call MPI_FINALIZE(ierr)

@@ -93,6 +93,9 @@ def read_output(path: str) -> Dict[str, np.ndarray]:
     out["b"], out["f"], out["res"] = reals(ntot), reals(ntot), reals(ntot)
     out["hist"] = reals(4 * L * ncyc).reshape(ncyc, L, 4)
     out["p_post"] = reals(n1)
+    nvel = int(ints(1)[0])
+    if nvel:
+        out["u_post"], out["v_post"], out["w_post"] = reals(nvel), reals(nvel), reals(nvel)
     assert pos == len(raw), (pos, len(raw))
     return out
 

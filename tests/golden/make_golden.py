@@ -16,6 +16,9 @@ injected, those arrays) and the reference's outputs.  Cases:
   g6_c0        configs[0] at full size (64x64x32, 4 levels, Dirichlet outflow, mean-free
                divergence source), 30 V(2,2): norm history, p after extrap_p + p_ref_subtract
                (strided sample + checksums)
+  g7_c4s8_uvw  as g2 with rho^{n+1} = rho (1 + 10% seeded noise), 3 V(2,2), then the velocity
+  g8_c1s8_uvw  correction uvw_adv (U, V, W) with the final p: p, rho^{n+1} and the corrected
+               u, v, w (2 and 4 blocks: interfaces, periodic z, walls)
 """
 from __future__ import annotations
 
@@ -42,6 +45,14 @@ def fields(p):
     us, vs, ws = P.velocities(p)
     rho = P.density(p)
     return ref_io.level1_fields(p, dict(u=us, v=vs, w=ws, rho=rho, rhonp1=rho))
+
+
+def uvw_fields(p):
+    """Fields of the velocity-correction fixtures: rho^{n+1} differs from rho (seeded)."""
+    fl = fields(p)
+    rng = np.random.default_rng(7)
+    fl["rhonp1"] = fl["rho"] * (1.0 + 0.1 * rng.random(fl["rho"].size))
+    return fl
 
 
 def save(name, out, extra=None, keys=FULL_KEYS):
@@ -98,6 +109,15 @@ def main():
     out = ref_io.run(p, ncycles=8, coeffs=a, b1=b1, n1=o.n1)
     save("g5_c4s8_ext", out, dict(cfg="c4", scale=8, ncycles=8, cycle=1, b1_in=b1,
                                    **{"in_" + k: v for k, v in zip("ap aw ae as an al ar".split(), a)}))
+
+    for name, cfg, sc in [("g7_c4s8_uvw", "c4", 8), ("g8_c1s8_uvw", "c1", 8)]:
+        p = P.config(cfg, sc)
+        fl = uvw_fields(p)
+        out = ref_io.run(p, ncycles=3, fields=fl, n1=fl["u"].size)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), cfg=cfg, scale=sc, ncycles=3,
+                            rhonp1=fl["rhonp1"], p_post=out["p_post"], u_post=out["u_post"],
+                            v_post=out["v_post"], w_post=out["w_post"])
+        print(f"{name}: {os.path.getsize(os.path.join(HERE, name + '.npz')) / 1024:.1f} KiB")
 
     p = P.config("c0", 1)
     o, a, b1 = ext_inputs(p)  # rho = 1 for c0: only the Dirichlet outflow differs from coeff_p

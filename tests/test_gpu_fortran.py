@@ -1,11 +1,13 @@
 """The Fortran drop-in over the C ABI.
 
 oracle/_ref/ref_driver_gpu is the reference's own pressure path (oracle/ref_driver.F with the
-reference objects built from code/*.F) in which code/lmg_solve.F is replaced by the product's
-fortranutils_amd/fortran/lmg_solve_gpu.F: the 12-argument lmg_solve reading LESter's common
-blocks and calling liblmgx through ISO_C_BINDING.  On the same input file the two drivers must
-write bitwise equal pressure (before and after extrap_p / p_ref_subtract), residual and
-right-hand sides on every level (the drop-in's default is the reference's own schedule); the
+reference objects built from code/*.F) in which code/lmg_solve.F and code/uvw_adv.F are replaced
+by the product's fortranutils_amd/fortran/lmg_solve_gpu.F (the 12-argument lmg_solve reading
+LESter's common blocks) and fortranutils_amd/fortran/uvw_adv_gpu.F (the 11-argument uvw_adv),
+both calling liblmgx through ISO_C_BINDING on one shared handle.  On the same input file the two
+drivers must write bitwise equal pressure (before and after extrap_p / p_ref_subtract), residual
+and right-hand sides on every level (the drop-in's default is the reference's own schedule) and
+bitwise equal corrected velocities u, v, w; the
 per-cycle norms, whose sums run in a different order on the GPU, agree to 1e-12 relative.
 V- and F-cycles.
 
@@ -20,7 +22,7 @@ import ref_io
 
 pytestmark = pytest.mark.gpu
 
-KEYS = ("f", "res", "b", "p_post")
+KEYS = ("f", "res", "b", "p_post", "u_post", "v_post", "w_post")
 
 
 def _fields(p):
