@@ -6,6 +6,8 @@
 #include "../../include/lmgx.h"
 
 #include <algorithm>
+#include <charconv>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -909,3 +911,4 @@ extern "C" int lmgx_profile_read(const lmgx_handle *hc, double *ms, long *launch
 #include "fast_path.inc"
 #include "operator_build.inc"
 #include "adjoint.inc"
+#include "field_io.inc"

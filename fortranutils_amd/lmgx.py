@@ -207,6 +207,22 @@ class Solver:
                                              _dp(p), C.c_double(dtime)))
         return u
 
+    # ---- field dumps (reference code/file_3.F, code/read_3.F) ------------------------------
+    def file_3(self, filename, f, filefrmt=0, level=0, mg_or_sg=1, real_kind=4):
+        """Write level ``level`` of the device field ``f`` (multi-level natural layout) in the
+        reference's 3-D file format: ``filefrmt`` 0 unformatted (REAL*4 values, or REAL*8 with
+        ``real_kind=8``), 1 text."""
+        _check(lib().lmgx_file_3_device(self.h, os.fsencode(filename), int(filefrmt), _dp(f),
+                                        int(level), int(mg_or_sg), int(real_kind)))
+
+    def read_3(self, filename, f, filefrmt=0, real_kind=4) -> bool:
+        """Read such a file into level 1 of the device field ``f``; a missing file gives zeros
+        and returns False, as the reference does."""
+        found = C.c_int()
+        _check(lib().lmgx_read_3_device(self.h, os.fsencode(filename), int(filefrmt), _dp(f),
+                                        int(real_kind), C.byref(found)))
+        return bool(found.value)
+
     # ---- solve --------------------------------------------------------------------------
     def lmg_solve(self, f, b, residu, *, max_cycles=5, history=True, **kw):
         """Run up to ``max_cycles`` V-cycles.  numpy arrays -> host entry point, torch CUDA

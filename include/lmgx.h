@@ -31,6 +31,7 @@ extern "C" {
 #define LMGX_E_STATE (-5)    /* call out of order (e.g. solve before set_operator) */
 #define LMGX_E_NUMERIC (-6)  /* NaN/Inf in the residual norms */
 #define LMGX_E_UNSUPPORTED (-7)
+#define LMGX_E_FILE (-8)     /* field dump: cannot open / read / write, or the file does not fit */
 
 /* boundary codes, reference code/bc_type.inc:17-27 (only these two change the solver) */
 #define LMGX_BC_PERIODIC 12
@@ -177,6 +178,23 @@ int lmgx_uvw_adv(lmgx_handle *h, int ivar, const double *diu, const double *rp, 
 int lmgx_uvw_adv_device(lmgx_handle *h, int ivar, const double *d_diu, const double *d_rp,
                         const double *d_dkw, double *d_u, const double *d_u_int,
                         const double *d_rho, const double *d_p, double dtime);
+
+/* Field dumps in the reference's 3-D file format (the restart / warm-start files of LESter).
+ * lmgx_file_3_device replaces code/file_3.F:5 file_3(dev, filename, filefrmt, f, ilev,
+ * mg_or_sg, ierr): writes the device field d_f (multi-level natural layout) of level `level`
+ * (0-based; mg_or_sg = 2, a single-grid field, means level 0) to `filename` as a Fortran
+ * sequential file - filefrmt 0: unformatted records (4-byte length markers), numbl, the
+ * (nip1, njp1, nkp1) of every block, then one record per (j, k) row of nip1 values, halos
+ * included, as REAL*4 (real_kind 4, the reference's default) or REAL*8 (real_kind 8, its
+ * -DWRITE_DOUBLE build); filefrmt 1: the same as text, each row written '(8(1x,g15.8))'.
+ * lmgx_read_3_device replaces code/read_3.F:5 read_3(dev, filename, filefrmt, f, work, ierr):
+ * reads such a file into level 0 of d_f.  A missing file gives zeros and *found = 0, as the
+ * reference does; a block count or block dimension that differs from the handle's, or a short
+ * record, returns LMGX_E_FILE (the message names the reference's ierr 3, 4 or 5). */
+int lmgx_file_3_device(lmgx_handle *h, const char *filename, int filefrmt, const double *d_f,
+                       int level, int mg_or_sg, int real_kind);
+int lmgx_read_3_device(lmgx_handle *h, const char *filename, int filefrmt, double *d_f,
+                       int real_kind, int *found);
 
 /* Adjoint (no counterpart in the reference; for gradients of pressure functionals).
  *

@@ -10,6 +10,7 @@ C   set_offset -> set_const -> set_loop_bound(SV_P) -> set_interfaces
 C   -> mg_xyz(x,y,z) -> comp_metric -> coeff_p -> sour_p
 C   -> lmg_solve (one call per V-cycle, norms logged after each)
 C   -> extrap_p -> p_ref_subtract -> uvw_adv (U, V, W)
+C   -> (optional) file_3 / read_3 of p
 C The commons are filled directly instead of through input.dat /
 C block.dat / x.bin, since only their values matter to the path.
 C
@@ -46,13 +47,14 @@ C=======================================================================
       integer nip1, njp1, nkp1, i, j, ijk, off, lwork, lgeo
       integer ijkse(6)
       real    t0, t1, tsolve
-      character*256 fin, fout
+      character*256 fin, fout, fdump
+      character*32 fnb, fnt, fnn
 
       real, allocatable :: ap(:), aw(:), ae(:), as(:), an(:), al(:),
      &     ar(:), f(:), b(:), res(:), f0(:)
       real, allocatable :: u(:), v(:), w(:), rho(:), rhonp1(:),
      &     drho(:), srhot(:), srhoux(:), gia(:), gja(:),
-     &     uo(:), vo(:), wo(:)
+     &     uo(:), vo(:), wo(:), frb(:), frt(:), frn(:)
       real, allocatable :: x(:), y(:), z(:), work(:), geo(:)
       real, allocatable :: m2(:,:), dkp(:), dkw(:)
       integer, allocatable :: pnti(:,:)
@@ -90,6 +92,9 @@ C COMMUNICATION AND CONTROL COMMONS (ONE RANK OWNS EVERY BLOCK)
         read(10) (bl_nijkp1(i,iblk),i=1,3), (bl_bc(i,iblk),i=1,6),
      &           (bl2bl(i,iblk),i=1,6), (bl2side(i,iblk),i=1,6)
         bl2pe(iblk) = 0
+        do i = 1,3
+          bl_nijkp1_master(i,iblk) = bl_nijkp1(i,iblk)
+        end do
       end do
 
       call set_offset()
@@ -278,6 +283,34 @@ C djv=10, rp=27 (V PASSES djv IN PLACE OF diu, AS step.F:421 DOES)
         write(11) uo, vo, wo
       else
         write(11) 0
+      end if
+
+C FIELD DUMPS (step.F:515, main.F:247), WHEN A THIRD ARGUMENT IS GIVEN:
+C file_3 OF THE FINAL p IN BOTH FORMATS INTO THE WORKING DIRECTORY
+C (p_bin: filefrmt 0, p_txt: filefrmt 1), THEN read_3 OF EACH, AND OF
+C A MISSING FILE (WHICH read_3 ANSWERS WITH ZEROS)
+
+      call get_command_argument(3, fdump)
+      if (len_trim(fdump) .gt. 0) then
+        allocate(frb(n1), frt(n1), frn(n1))
+        frb = -1.
+        frt = -1.
+        frn = -1.
+        ierr = 0
+C       file names as LESter passes them: blank-padded character*32
+C       (namvar, cb.control.inc:43), which read_3's extr_string scans
+        fnb = 'p_bin'
+        fnt = 'p_txt'
+        fnn = 'p_none'
+        call file_3(20,fnb,0,f,1,1,ierr)
+        call file_3(21,fnt,1,f,1,1,ierr)
+        call read_3(22,fnb,0,frb,work,ierr)
+        call read_3(23,fnt,1,frt,work,ierr)
+        call read_3(24,fnn,0,frn,work,ierr)
+        write(11) n1, ierr
+        write(11) frb, frt, frn
+      else
+        write(11) 0, 0
       end if
       close(11)
 

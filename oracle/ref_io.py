@@ -96,13 +96,21 @@ def read_output(path: str) -> Dict[str, np.ndarray]:
     nvel = int(ints(1)[0])
     if nvel:
         out["u_post"], out["v_post"], out["w_post"] = reals(nvel), reals(nvel), reals(nvel)
+    ndump, out["dump_ierr"] = (int(x) for x in ints(2))
+    if ndump:
+        out["p_read_bin"], out["p_read_txt"], out["p_read_none"] = (reals(ndump), reals(ndump),
+                                                                     reals(ndump))
     assert pos == len(raw), (pos, len(raw))
     return out
 
 
 def run(problem, *, ncycles: int, coeffs=None, b1=None, fields=None, f0=None, n1=None,
         do_post: int = 1, ntime: int = 1, tolf: float = 0.0, timeout: float = 3600.0,
-        cycle: int = 1, gpu: bool = False, env_extra: Optional[dict] = None):
+        cycle: int = 1, gpu: bool = False, env_extra: Optional[dict] = None,
+        dump: bool = False):
+    """dump: also write the final p with the reference's file_3 (binary and text, in the
+    driver's working directory) and read both back with read_3; the files' bytes are returned
+    as out["file_p_bin"] / out["file_p_txt"]."""
     if not available(gpu):
         raise RuntimeError("reference driver not built: make -C oracle ref ref-gpu")
     with tempfile.TemporaryDirectory() as td:
@@ -111,11 +119,17 @@ def run(problem, *, ncycles: int, coeffs=None, b1=None, fields=None, f0=None, n1
                     n1=n1, do_post=do_post, ntime=ntime, tolf=tolf, cycle=cycle)
         env = dict(os.environ)
         env.update(env_extra or {})
-        r = subprocess.run([DRIVER_GPU if gpu else DRIVER, fin, fout], capture_output=True,
-                           text=True, env=env, timeout=timeout)
+        args = [DRIVER_GPU if gpu else DRIVER, fin, fout] + (["dump"] if dump else [])
+        r = subprocess.run(args, capture_output=True, text=True, errors="replace", env=env,
+                           timeout=timeout, cwd=td)
         if r.returncode != 0 or not os.path.exists(fout):
             raise RuntimeError(f"ref_driver failed ({r.returncode}): {r.stdout}\n{r.stderr}")
-        return read_output(fout)
+        out = read_output(fout)
+        if dump:
+            for name in ("p_bin", "p_txt"):
+                with open(os.path.join(td, name), "rb") as fh:
+                    out["file_" + name] = fh.read()
+        return out
 
 
 def level1_fields(problem, fields_per_block: dict) -> dict:

@@ -19,6 +19,11 @@ injected, those arrays) and the reference's outputs.  Cases:
   g7_c4s8_uvw  as g2 with rho^{n+1} = rho (1 + 10% seeded noise), 3 V(2,2), then the velocity
   g8_c1s8_uvw  correction uvw_adv (U, V, W) with the final p: p, rho^{n+1} and the corrected
                u, v, w (2 and 4 blocks: interfaces, periodic z, walls)
+  g9_c4s8_dump file_3 of the final p of 2 V(2,2) on c4/8 in both formats (the files' bytes), and
+               what read_3 gives back for each and for a missing file
+  g10_c4s8_dump_special  the same for a crafted field (no solve): seeded magnitudes 1e-30..1e30
+               and the edge cases of G15.8 and of the REAL*4 conversion (zeros, rounding carries,
+               3-digit exponents, subnormals, overflow, Inf, NaN)
 """
 from __future__ import annotations
 
@@ -53,6 +58,30 @@ def uvw_fields(p):
     rng = np.random.default_rng(7)
     fl["rhonp1"] = fl["rho"] * (1.0 + 0.1 * rng.random(fl["rho"].size))
     return fl
+
+
+SPECIAL = [0.0, -0.0, 1e-100, -1e-100, 1e100, 9.999999995, 9.9999999949, 0.099999999,
+           0.0999999995, 0.1, 1.0, 99999999.5, 99999999.4, 1e8, 1e7, 12345678.9, 1.5e-5,
+           123.456789012, -5e-324, 3.4028235e38, 1e-45, 0.999999995, 0.99999999, 1e99,
+           9.99999999e99, 1e-99, 1.00000001e-99, 0.5, 2.5, 1234567.85, 1e-10, 7.0,
+           float("inf"), float("-inf"), float("nan"), 1e300, -3.5e38, 1.17549435e-38]
+
+
+def special_field(n):
+    """Seeded log-uniform magnitudes 1e-30..1e30 of random sign, the edge cases first."""
+    rng = np.random.default_rng(5)
+    f = np.sign(rng.standard_normal(n)) * 10.0 ** rng.uniform(-30, 30, n)
+    f[: len(SPECIAL)] = SPECIAL
+    return f
+
+
+def save_dump(name, out, field):
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), cfg="c4", scale=8, field=field,
+                        file_bin=np.frombuffer(out["file_p_bin"], np.uint8),
+                        file_txt=np.frombuffer(out["file_p_txt"], np.uint8),
+                        read_bin=out["p_read_bin"], read_txt=out["p_read_txt"],
+                        read_none=out["p_read_none"])
+    print(f"{name}: {os.path.getsize(os.path.join(HERE, name + '.npz')) / 1024:.1f} KiB")
 
 
 def save(name, out, extra=None, keys=FULL_KEYS):
@@ -118,6 +147,16 @@ def main():
                             rhonp1=fl["rhonp1"], p_post=out["p_post"], u_post=out["u_post"],
                             v_post=out["v_post"], w_post=out["w_post"])
         print(f"{name}: {os.path.getsize(os.path.join(HERE, name + '.npz')) / 1024:.1f} KiB")
+
+    p = P.config("c4", 8)
+    fl = fields(p)
+    n1 = fl["u"].size
+    out = ref_io.run(p, ncycles=2, fields=fl, n1=n1, dump=True)
+    save_dump("g9_c4s8_dump", out, field=out["p_post"])
+    f0 = special_field(n1)
+    out = ref_io.run(p, ncycles=0, fields=fl, n1=n1, f0=f0, do_post=0, dump=True)
+    assert np.array_equal(out["p_post"], f0, equal_nan=True)
+    save_dump("g10_c4s8_dump_special", out, field=f0)
 
     p = P.config("c0", 1)
     o, a, b1 = ext_inputs(p)  # rho = 1 for c0: only the Dirichlet outflow differs from coeff_p
