@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--pmc-summary", nargs="+", metavar="CSV")
     ap.add_argument("--stats-summary", metavar="CSV", help="markdown table of a rocprofv3 kernel_stats.csv")
     ap.add_argument("--traffic", metavar="DIR")
+    ap.add_argument("--step-parts", action="store_true",
+                    help="also time the parts of a projection step around the solve (density "
+                         "update, source, velocity correction, field dump)")
     return ap.parse_args()
 
 
@@ -115,6 +118,50 @@ def solve(s, b, args, mode, rtol, profile=False):
         s.profile(False)
         return f, hist, (ms, n, cells)
     return f, hist, None
+
+
+def step_parts(p, s, args, reps=5):
+    """Wall time (ms, mean of reps; every library call synchronises its stream) of the parts of
+    one projection step of the LES time step (code/step.F:353-425) on the resident problem:
+    coefficients for a new density, source, the solve, extrap_p + p_ref_subtract, the three
+    velocity corrections; and a REAL*4 binary dump of p (code/file_3.F) to local disk."""
+    import tempfile
+    import torch
+    from fortranutils_amd import problems as P
+    fl = P.device_fields(p)
+    n1 = s.n1
+
+    def clock(fn, n=reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / n
+
+    b = torch.zeros(s.ntot, dtype=torch.float64, device="cuda")
+    f = torch.zeros_like(b)
+    uo = fl["u"].clone()
+    out = {}
+    out["update_density"] = clock(lambda: s.update_density(fl["rhonp1"]))
+    out["source"] = clock(lambda: s.source(fl["u"], fl["v"], fl["w"], fl["rho"], fl["rhonp1"],
+                                           p.dtime, b))
+    out["solve_same_operator"] = clock(lambda: solve(s, b, args, "mixed", args.rtol))
+
+    def new_op_solve():
+        s.update_density(fl["rhonp1"])
+        solve(s, b, args, "mixed", args.rtol)
+    out["update_density+solve"] = clock(new_op_solve)
+    out["extrap_p+p_ref_subtract"] = clock(lambda: (s.extrap_p(f), s.p_ref_subtract(f)))
+    out["uvw_adv (one component)"] = clock(lambda: s.uvw_adv(1, uo, fl["u"], fl["rhonp1"], f,
+                                                             p.dtime))
+    with tempfile.TemporaryDirectory() as td:
+        out["file_3 binary REAL*4"] = clock(lambda: s.file_3(os.path.join(td, "p"), f), n=2)
+    for k, v in out.items():
+        print(f"# step part {k}: {v:.2f} ms", file=sys.stderr, flush=True)
+    del fl
+    return out
 
 
 def timed(s, b, args, mode, steps, warmup):
@@ -370,6 +417,8 @@ def main():
         "cpu_baseline": cpu,
     }
     out.update(extra)
+    if args.step_parts:
+        out["step_parts_ms"] = step_parts(p, s, args)
     print(json.dumps(out))
 
 

@@ -177,6 +177,11 @@ class Solver:
                                          None if r is None else _dp(r),
                                          None if d is None else _ip(d)))
 
+    def update_density(self, rho1=None):
+        """Rebuild the coefficients of every level on the device for a new level-1 density (a
+        device tensor of n1 with halos; None = 1), keeping the metrics and Dirichlet sides."""
+        _check(lib().lmgx_update_density_device(self.h, None if rho1 is None else _dp(rho1)))
+
     def set_dirichlet(self, flags):
         f = None if flags is None else np.ascontiguousarray(flags, np.int32).ravel()
         _check(lib().lmgx_set_dirichlet(self.h, None if f is None else _ip(f)))

@@ -153,6 +153,13 @@ int lmgx_prolong_device(lmgx_handle *h, int fine_level, double *d_f);
  * none).  The operator is stored in the handle (as lmgx_set_operator would). */
 int lmgx_build_operator(lmgx_handle *h, const double *xv1, const double *yv1, const double *zv1,
                         const double *rho, const int *dirichlet);
+/* Per-step density change (the variable-density LES step): the coefficients of every level
+ * rebuilt on the device from the metrics kept by lmgx_build_operator, for a new level-1 density
+ * d_rho1 (device, n1 doubles with halos; NULL = 1), with the face-Dirichlet sides already set.
+ * The same operator as lmgx_build_operator with that density, without the geometry pass and
+ * the host transfer (reference: coeff_p, code/coeff_p.F:5-78, is re-run whenever the grid
+ * changes; the density scaling has no reference counterpart). */
+int lmgx_update_density_device(lmgx_handle *h, const double *d_rho1);
 /* Copy the handle's operator to host (7 x ntot) - for checking and for hand-off to LESter. */
 int lmgx_get_operator(const lmgx_handle *h, double *ap, double *aw, double *ae, double *as,
                       double *an, double *al, double *ar);

@@ -36,3 +36,43 @@ def test_build_operator_bitwise(cfg, scale, ext):
              p.dtime, b)
     bo = o.sour(geo, fl["u"], fl["v"], fl["w"], fl["rho"], fl["rhonp1"], p.dtime)
     assert np.array_equal(b.cpu().numpy()[: o.n1], bo[: o.n1])
+
+
+@pytest.mark.parametrize("cfg,scale", [("c4", 8), ("c3", 8)])
+def test_update_density_matches_build(cfg, scale):
+    """lmgx_update_density_device (coefficients rebuilt on the device for a new density) gives
+    bitwise the operator of lmgx_build_operator with that density, and of the oracle; a fast
+    solve after the update equals a fast solve on a freshly built operator bitwise."""
+    import torch
+    from fortranutils_amd import lmgx
+    p = P.config(cfg, scale)
+    o = O.Oracle(p)
+    geo = o.geometry()
+    fl = fields(p, o)
+    dirich = np.array([b.dirichlet for b in p.blocks], np.int32)
+    rng = np.random.default_rng(2)
+    rho2 = fl["rho"] * (1.0 + 0.2 * rng.random(o.n1))
+    a = lmgx.Solver(lmgx.Topology.from_problem(p))
+    a.build_operator(p, rho1=fl["rho"], dirichlet=dirich)
+    a.update_density(torch.from_numpy(rho2).cuda())
+    b = lmgx.Solver(lmgx.Topology.from_problem(p))
+    b.build_operator(p, rho1=rho2, dirichlet=dirich)
+    ref = o.coeff(geo, rho=o.rho_levels(rho2), dirichlet=dirich)
+    for k, x, y, z in zip("ap aw ae as an al ar".split(), a.get_operator(), b.get_operator(), ref):
+        assert np.array_equal(x, y) and np.array_equal(x, z), k
+    # density 1 again: the reference's coeff_p with the Dirichlet sides
+    a.update_density(None)
+    ref1 = o.coeff(geo, dirichlet=dirich)
+    for k, x, z in zip("ap aw ae as an al ar".split(), a.get_operator(), ref1):
+        assert np.array_equal(x, z), k
+    a.update_density(torch.from_numpy(rho2).cuda())
+    bsrc = torch.from_numpy(o.sour(geo, fl["u"], fl["v"], fl["w"], fl["rho"], rho2, p.dtime)).cuda()
+    out = []
+    for s in (a, b):
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, bsrc.clone(), res, smoother=1, precision=1, rtol=1e-10, tolf=0.0,
+                           max_cycles=30, numpre=1, numpost=1, sweeps_per_iter=3, omega=1.35)
+        out.append((f.cpu().numpy(), hist))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert len(out[0][1]) < 30                  # reached 1e-10 before the cycle cap
