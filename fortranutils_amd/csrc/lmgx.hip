@@ -87,6 +87,7 @@ struct lmgx_handle {
     long swap_cap = 0;
     double *d_partial = nullptr;
     long partial_cap = 0;
+    double *d_partial2 = nullptr;       // middle-stage partials of long reductions
     double *d_scalar = nullptr;         // 64 doubles of small device scratch
     double *d_hist = nullptr;           // [cap][nlev][4]
     long hist_cap = 0;
@@ -342,6 +343,7 @@ static int upload_tables(lmgx_handle *h)
     h->swap_cap = swap_need;
     HIPCHK(hipMalloc(&h->d_partial, sizeof(double) * part_need));
     h->partial_cap = part_need;
+    HIPCHK(hipMalloc(&h->d_partial2, sizeof(double) * 4 * LMGX_RMID));
     HIPCHK(hipMalloc(&h->d_scalar, sizeof(double) * 64));
     return 0;
 }
@@ -378,7 +380,8 @@ extern "C" int lmgx_free(lmgx_handle *h)
     for (int q = 0; q < 7; ++q) { hipFree(h->d_a[q]); hipFree(h->d_aT[q]); }
     hipFree(h->d_rho_op);
     hipFree(h->d_f); hipFree(h->d_b); hipFree(h->d_res);
-    hipFree(h->d_swap); hipFree(h->d_partial); hipFree(h->d_scalar); hipFree(h->d_hist);
+    hipFree(h->d_swap); hipFree(h->d_partial); hipFree(h->d_partial2); hipFree(h->d_scalar);
+    hipFree(h->d_hist);
     hipFree(h->d_met); hipFree(h->d_dkp); hipFree(h->d_dkw);
     if (h->h_pinned) hipHostFree(h->h_pinned);
     free_fast(h);
@@ -525,6 +528,23 @@ static int ensure_hist(lmgx_handle *h, long n)
     return 0;
 }
 
+// The 4 norms from nparts workgroup partials in h->d_partial (RMS with the reference's var = 0 -> 1
+// rule) into d_out: long lists go through LMGX_RMID middle-stage workgroups first, so no single
+// workgroup walks tens of thousands of partials.
+static void reduce_norms(lmgx_handle *h, long nparts, double scale, double *d_out)
+{
+    if (nparts <= 4 * LMGX_WG) {
+        hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), dim3(LMGX_WG), 0, h->stream,
+                           (const double *)h->d_partial, nparts, scale, d_out, 0);
+        return;
+    }
+    const long chunk = (nparts + LMGX_RMID - 1) / LMGX_RMID;
+    hipLaunchKernelGGL(k_reduce_mid<4>, dim3(LMGX_RMID), dim3(LMGX_WG), 0, h->stream,
+                       (const double *)h->d_partial, nparts, chunk, h->d_partial2);
+    hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), dim3(LMGX_WG), 0, h->stream,
+                       (const double *)h->d_partial2, (long)LMGX_RMID, scale, d_out, 0);
+}
+
 // norms of level l into d_out[4] (device), residual into res
 static int norms_exact(lmgx_handle *h, int l, const double *x, const double *b, double *res,
                        double *d_out)
@@ -533,8 +553,7 @@ static int norms_exact(lmgx_handle *h, int l, const double *x, const double *b, 
     double *const *a = h->d_a;
     hipLaunchKernelGGL(k_norms_exact, dim3(D.tiles_int), dim3(LMGX_WG), 0, h->stream, D.d_lb,
                        h->nblk, a[0], a[1], a[2], a[3], a[4], a[5], a[6], x, b, res, h->d_partial);
-    hipLaunchKernelGGL(k_reduce_final<4>, dim3(1), dim3(LMGX_WG), 0, h->stream, h->d_partial,
-                       D.tiles_int, 1.0 / h->npts[l], d_out, 0);
+    reduce_norms(h, D.tiles_int, 1.0 / h->npts[l], d_out);
     HIPCHK(hipGetLastError());
     return 0;
 }

@@ -6,6 +6,7 @@
 #define LMGX_MAXB 64
 #define LMGX_MAXLEV 8
 #define LMGX_WG 256
+#define LMGX_RMID 256   // middle-stage workgroups of a long norm reduction
 #define LMGX_RCH 1      // coarse rows per thread of the residual column kernel
 #define LMGX_RCHP 8     // coarse rows per thread of the prolongation column kernel
 #define LMGX_SMAX 16    // segmented line solve: rows per segment held in registers, 16 segments

@@ -48,9 +48,12 @@ def test_fast_converges_to_direct_solution(cfg, scale, prec):
     x = f.cpu().numpy()[: o.n1][m]
     err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
     assert err < 1e-9, err
-    # the returned level-1 residual is the residual of the returned solution
+    # the returned level-1 residual is the residual of the returned solution: b - A f to the
+    # rounding of two FP64 evaluations in different orders
     r = res.cpu().numpy()[: o.n1][m]
     assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(bh[m])
+    scale = np.abs(bh[m]) + abs(A) @ np.abs(x)
+    assert np.all(np.abs(r - (bh[m] - A @ x)) <= 1e-13 * scale)
 
 
 @pytest.mark.parametrize("prec", [0, 1])
