@@ -7,7 +7,9 @@
 #define LMGX_MAXLEV 8
 #define LMGX_WG 256
 #define LMGX_RMID 256   // middle-stage workgroups of a long norm reduction
+#ifndef LMGX_RCH
 #define LMGX_RCH 1      // coarse rows per thread of the residual column kernel
+#endif
 #define LMGX_RCHP 8     // coarse rows per thread of the prolongation column kernel
 #define LMGX_SMAX 16    // segmented line solve: rows per segment held in registers, 16 segments
                         // at most, so lines up to 256 rows
