@@ -57,10 +57,15 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 GPU path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--no-stream-ceiling", action="store_true",
+                    help="skip the measured streaming rate of the box (torch kernels)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--pmc-summary", nargs="+", metavar="CSV")
     ap.add_argument("--stats-summary", metavar="CSV", help="markdown table of a rocprofv3 kernel_stats.csv")
+    ap.add_argument("--stats-diff", nargs=3, metavar=("CSV_A", "CSV_B", "N"),
+                    help="per-solve kernel time: (B - A) / N for two kernel_stats.csv of runs that "
+                         "differ by N timed solves")
     ap.add_argument("--traffic", metavar="DIR")
     ap.add_argument("--step-parts", action="store_true",
                     help="also time the parts of a projection step around the solve (density "
@@ -77,6 +82,34 @@ def copy_calib():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calib_bytes": 4 * n, "dispatches": 3}))
+
+
+def stream_ceiling(reps=5):
+    """What plain streaming kernels reach on this GPU, as the practical ceiling next to the
+    published 8 TB/s: torch's column sum of a (9, n) FP32 tensor (9 words read, 1 written per
+    element, the smoother's word count) and a device copy, n = 62.9M (2.5 GB touched), best of
+    ``reps`` launches timed with events on torch's stream."""
+    import torch
+    n = 60 << 20
+    a = torch.ones((9, n), dtype=torch.float32, device="cuda")
+    y = torch.empty(n, dtype=torch.float32, device="cuda")
+    c = torch.empty_like(a)
+    out = {}
+    for name, fn, nbytes in (("sum9_GBps", lambda: torch.sum(a, dim=0, out=y), 40 * n),
+                             ("copy_GBps", lambda: c.copy_(a), 72 * n)):
+        fn()
+        best = float("inf")
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+        out[name] = nbytes / best / 1e9
+    del a, y, c
+    torch.cuda.empty_cache()
+    return out
 
 
 def make_problem(args):
@@ -274,6 +307,35 @@ def stats_summary(path, top=25):
     print(f"\n{len(rows)} kernels, {tot / 1e6:.1f} ms in total")
 
 
+def stats_diff(pa, pb, n, top=30):
+    """Kernel time per solve from two rocprofv3 --stats runs of the same workload that differ only
+    in the number of timed solves (by n): setup, tuning and profiling cancel."""
+    import csv
+
+    def load(path):
+        out = {}
+        for r in csv.DictReader(open(path)):
+            name = r["Name"].replace("void ", "").split("(")[0].replace("|", "/")
+            c, t = out.get(name, (0, 0.0))
+            out[name] = (c + int(r["Calls"]), t + float(r["TotalDurationNs"]))
+        return out
+    a, b = load(pa), load(pb)
+    rows = []
+    for k, (cb, tb) in b.items():
+        ca, ta = a.get(k, (0, 0.0))
+        dc, dt = (cb - ca) / n, (tb - ta) / n / 1e6
+        if dc > 0 or dt > 1e-3:
+            rows.append((dt, dc, k))
+    rows.sort(reverse=True)
+    tot = sum(r[0] for r in rows)
+    print("| kernel | calls per solve | ms per solve | avg us | % |")
+    print("|---|---:|---:|---:|---:|")
+    for dt, dc, k in rows[:top]:
+        print(f"| `{k[:90]}` | {dc:.1f} | {dt:.3f} | {1e3 * dt / max(dc, 1):.1f} | "
+              f"{100 * dt / tot:.1f} |")
+    print(f"\n{tot:.2f} ms of kernel time per solve")
+
+
 def traffic(d):
     """profiles/traffic.json entry from four rocprofv3 --pmc runs under directory ``d``:
     copy_FETCH_SIZE/, copy_WRITE_SIZE/ (``--copy-calib``: 3 copies of 1 GiB) and
@@ -327,6 +389,9 @@ def main():
         return
     if args.stats_summary:
         stats_summary(args.stats_summary)
+        return
+    if args.stats_diff:
+        stats_diff(args.stats_diff[0], args.stats_diff[1], int(args.stats_diff[2]))
         return
     if args.traffic:
         traffic(args.traffic)
@@ -385,6 +450,13 @@ def main():
                     cells_per_launch=cells / max(nl, 1))
         print(f"# smoother: {nl} launches, avg {1e6 * avg_s:.1f} us, {achieved / 1e9:.0f} GB/s, "
               f"tile widths fp64/fp32 {s.tile_widths().tolist()}", file=sys.stderr, flush=True)
+        if not args.no_stream_ceiling:
+            sc = stream_ceiling()
+            roof["stream_ceiling"] = sc
+            roof["frac_of_stream"] = achieved / 1e9 / max(sc.values())
+            print(f"# streaming on this GPU: 9-read/1-write {sc['sum9_GBps']:.0f} GB/s, copy "
+                  f"{sc['copy_GBps']:.0f} GB/s; smoother at {roof['frac_of_stream']:.2f} of the "
+                  f"faster", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
