@@ -441,7 +441,7 @@ def main():
         tm = int(s.tile_widths()[1 if word == 4 else 0][0])
         rl = "true" if args.omega != 1.0 else "false"
         fp = "float" if word == 4 else "double"
-        key = (f"k_zebra_seg<{fp}, 1, {rl}>" if tm == 16 else f"k_zebra_line<{fp}, 1, {tm}, {rl}>")
+        key = (f"k_zebra_seg<{fp}, 1, {rl}, 0>" if tm == 16 else f"k_zebra_line<{fp}, 1, {tm}, {rl}>")
         tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=achieved / PEAK_HBM, traffic=tr,

@@ -223,8 +223,10 @@ int lmgx_rho_adjoint_device(lmgx_handle *h, const double *d_lambda, const double
  * FP64, 1 = FP32; 0 for levels that do not exist or before the first fast solve. */
 int lmgx_tile_widths(const lmgx_handle *h, int *out);
 
-/* Kernel timing of the last solve: average duration (ms) and count of the dominant smoother
- * kernel, measured with HIP events on the handle's stream. */
+/* Kernel timing of the last solve: total duration (ms), count and updated cells of the dominant
+ * smoother kernel, measured with HIP events on the handle's stream.  The fast path times the
+ * finest level's plain half-sweeps (not the two zero-start half-sweeps that open each cycle's
+ * correction, which read fewer words). */
 int lmgx_profile(lmgx_handle *h, int enable);
 int lmgx_profile_read(const lmgx_handle *h, double *smoother_ms_total, long *smoother_launches,
                       double *smoother_cells_total);
