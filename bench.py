@@ -112,6 +112,45 @@ def stream_ceiling(reps=5):
     return out
 
 
+def solve_bytes(s, args, mode, cycles):
+    """Algorithmic HBM bytes of one solve of ``cycles`` cycles with this schedule, per kernel and
+    level (halo swaps, O(surface), count 0; symmetric operator, as the solver checks):
+      half-sweep: 10 words per updated cell (9 with omega = 1); the two zero-start half-sweeps
+        that open a level's correction on the segmented kernel read 8 and 9 words;
+      zeroing a level's correction where no zero start applies: 1 word per cell;
+      restriction: residual from b, x, ap, ae, an, ar (6 words per fine cell) + the coarse b;
+      prolongation + correction: fine x read and written + coarse x;
+      coarsest direct solve: the dense inverse (n^2 FP64) and two vectors;
+      level 1, mixed: FP64 residual + norms (6 FP64 words) writing the FP32 right-hand side, and
+        x64 += e32 (FP64 x read and written, e read); FP64: the residual without the rhs;
+      per solve: f and b to the split layout, the first residual, the solution back and its
+        natural-layout residual (ap, ae, an, ar, b, x read, r written)."""
+    L = s.nlev
+    N = [float(np.prod(np.asarray(s.nijkp1[l]) - 2, axis=1).sum()) for l in range(L)]
+    mixed = mode == "mixed"
+    w = [4 if mixed and l < L - 1 else 8 for l in range(L)]
+    hw = 10 if args.omega != 1.0 else 9
+    tw = s.tile_widths()[1 if mixed else 0]
+    npre = args.numpre * args.spi
+    cyc = 0.0
+    for l in range(L - 1):
+        cyc += (args.numpre + args.numpost) * args.spi * N[l] * hw * w[l]
+        if mixed and npre >= 1 and int(tw[l]) == 16:
+            cyc -= 0.5 * N[l] * ((hw - 8) + (hw - 9)) * w[l]     # zero-start half-sweeps
+        elif mixed or l > 0:
+            cyc += N[l] * w[l]                                     # zeroed before use
+        cyc += 6 * N[l] * w[l] + N[l + 1] * w[l + 1]               # residual + restriction
+        cyc += 2 * N[l] * w[l] + N[l + 1] * w[l + 1]               # prolongation + correction
+    nc = N[L - 1]
+    cyc += 8 * (nc * nc + 2 * nc)                                  # coarsest direct solve
+    n1 = N[0]
+    cyc += n1 * (6 * 8 + (4 if mixed else 0))                      # FP64 residual + norms
+    if mixed:
+        cyc += n1 * 20                                             # x64 += e32
+    setup = n1 * (32 + 6 * 8 + (4 if mixed else 0) + 16 + 56)
+    return cycles * cyc + setup
+
+
 def make_problem(args):
     import torch
     from fortranutils_amd import lmgx, problems as P
@@ -419,7 +458,9 @@ def main():
     if not args.no_compare and args.mode != "fp64":
         dt64, cyc64, f64, h64 = timed(s, b, args, "fp64", 1, 0)
         err = (torch.linalg.norm(f[: s.n1] - f64[: s.n1]) / torch.linalg.norm(f64[: s.n1])).item()
+        sb64 = solve_bytes(s, args, "fp64", cyc64)
         extra = dict(fp64_ms_per_solve=1e3 * dt64, fp64_cycles=cyc64,
+                     fp64_solve_roofline_frac=sb64 / dt64 / PEAK_HBM,
                      speedup_vs_fp64=dt64 / (dt / args.steps),
                      p_rel_l2_vs_fp64=err)
         print(f"# fp64: {cyc64} cycles, {1e3 * dt64:.1f} ms; speed-up {extra['speedup_vs_fp64']:.2f}x,"
@@ -465,6 +506,15 @@ def main():
         except Exception as e:
             print("cpu baseline failed:", e, file=sys.stderr)
 
+    # the whole solve against the HBM roofline: algorithmic bytes of the schedule / time
+    sb = solve_bytes(s, args, args.mode, cycles / args.steps) if args.mode != "exact" else None
+    solve_roof = None
+    if sb:
+        ach = sb / (dt / args.steps)
+        solve_roof = dict(bytes_per_solve=sb, bytes_per_cell=sb / n1cells, achieved=ach / 1e9,
+                          unit="GB/s", frac=ach / PEAK_HBM)
+        print(f"# whole solve: {sb / n1cells:.0f} algorithmic B per fine cell, {ach / 1e9:.0f} GB/s "
+              f"= {ach / PEAK_HBM:.2f} of 8 TB/s", file=sys.stderr, flush=True)
     out = {
         "metric": "Mcell\u00b7V-cycles/s + time to 1e-10 residual, mixed-prec vs FP64, 1 GPU",
         "value": value,
@@ -486,6 +536,7 @@ def main():
         "time_to_rtol_ms": ms_step,
         "cycles_per_solve": cycles / args.steps,
         "roofline": roof,
+        "solve_roofline": solve_roof,
         "cpu_baseline": cpu,
     }
     out.update(extra)
