@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--omega-c", type=float, default=1.0)
     ap.add_argument("--max-cycles", type=int, default=100)
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 GPU path")
+    ap.add_argument("--compare-per-block", action="store_true",
+                    help="also time the same solve with the level work launched block by block "
+                         "(a per-block launch loop) instead of once per level for all blocks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--no-stream-ceiling", action="store_true",
@@ -465,6 +468,17 @@ def main():
                      p_rel_l2_vs_fp64=err)
         print(f"# fp64: {cyc64} cycles, {1e3 * dt64:.1f} ms; speed-up {extra['speedup_vs_fp64']:.2f}x,"
               f" |p_mixed - p_fp64|/|p_fp64| = {err:.2e}", file=sys.stderr, flush=True)
+
+    if args.compare_per_block and args.mode != "exact":
+        os.environ["LMGX_PER_BLOCK"] = "1"
+        dtpb, cycpb, fpb, _ = timed(s, b, args, args.mode, args.steps, 1)
+        os.environ["LMGX_PER_BLOCK"] = "0"
+        same = bool(torch.equal(fpb, f))
+        extra.update(per_block_ms_per_solve=1e3 * dtpb / args.steps,
+                     batched_speedup_vs_per_block=dtpb / dt, per_block_same_pressure=same)
+        print(f"# per-block launch loop: {1e3 * dtpb / args.steps:.1f} ms/solve, batched form "
+              f"{dtpb / dt:.2f}x faster, same pressure bitwise: {same}", file=sys.stderr,
+              flush=True)
 
     # dominant kernel: the finest-level line-relaxation half-sweep; HIP events on the solver's
     # stream.  Algorithmic bytes per updated cell, whatever the kernel: the right-hand side b,

@@ -52,6 +52,10 @@ static inline unsigned nblocks(long n) { return (unsigned)((n + LMGX_WG - 1) / L
 struct LevelDev {
     std::vector<LevBlk> h_lb;
     LevBlk *d_lb = nullptr;
+    // per-block launches (the fast path's block-loop mode): d_lb1[b] is block b alone, its tile
+    // offsets rebased to 0; bt_*[b] its tile counts in each tiling
+    LevBlk *d_lb1 = nullptr;
+    std::vector<long> bt_sg, bt_z64, bt_z128, bt_rr, bt_rp;
     long tiles_int = 0, tiles_all = 0, tiles_rb = 0, tiles_ln = 0, nlines = 0;
     long tiles_z64 = 0, tiles_z128 = 0, tiles_rr = 0, tiles_rp = 0, tiles_sg = 0;
     int njmax = 0;                      // longest j-line (interior rows) of the level
@@ -323,13 +327,36 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
     return 0;
 }
 
+// the level's block table, and every block alone with its tile offsets rebased to 0
+static int copy_tables(lmgx_handle *h, int l)
+{
+    LevelDev &D = h->lev[l];
+    HIPCHK(hipMemcpy(D.d_lb, D.h_lb.data(), sizeof(LevBlk) * h->nblk, hipMemcpyHostToDevice));
+    std::vector<LevBlk> one(D.h_lb);
+    for (LevBlk &B : one)
+        B.t_int = B.t_all = B.t_rb = B.t_ln = B.t_z64 = B.t_z128 = B.t_rr = B.t_rp = B.t_sg = 0;
+    HIPCHK(hipMemcpy(D.d_lb1, one.data(), sizeof(LevBlk) * h->nblk, hipMemcpyHostToDevice));
+    return 0;
+}
+
 static int upload_tables(lmgx_handle *h)
 {
     long swap_need = 1, part_need = 64;
     for (int l = 0; l < h->nlev; ++l) {
         LevelDev &D = h->lev[l];
         HIPCHK(hipMalloc(&D.d_lb, sizeof(LevBlk) * h->nblk));
-        HIPCHK(hipMemcpy(D.d_lb, D.h_lb.data(), sizeof(LevBlk) * h->nblk, hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&D.d_lb1, sizeof(LevBlk) * h->nblk));
+        CHK(copy_tables(h, l));
+        auto span = [&](long LevBlk::*f, long total, std::vector<long> &out) {
+            out.resize(h->nblk);
+            for (int b = 0; b < h->nblk; ++b)
+                out[b] = (b + 1 < h->nblk ? D.h_lb[b + 1].*f : total) - D.h_lb[b].*f;
+        };
+        span(&LevBlk::t_sg, D.tiles_sg, D.bt_sg);
+        span(&LevBlk::t_z64, D.tiles_z64, D.bt_z64);
+        span(&LevBlk::t_z128, D.tiles_z128, D.bt_z128);
+        span(&LevBlk::t_rr, D.tiles_rr, D.bt_rr);
+        span(&LevBlk::t_rp, D.tiles_rp, D.bt_rp);
         if (D.nfaces) {
             HIPCHK(hipMalloc(&D.d_fc, sizeof(Face) * D.nfaces));
             HIPCHK(hipMemcpy(D.d_fc, D.h_fc.data(), sizeof(Face) * D.nfaces, hipMemcpyHostToDevice));
@@ -375,6 +402,7 @@ extern "C" int lmgx_free(lmgx_handle *h)
     if (!h) return 0;
     for (int l = 0; l < h->nlev; ++l) {
         hipFree(h->lev[l].d_lb);
+        hipFree(h->lev[l].d_lb1);
         hipFree(h->lev[l].d_fc);
     }
     for (int q = 0; q < 7; ++q) { hipFree(h->d_a[q]); hipFree(h->d_aT[q]); }
@@ -439,9 +467,7 @@ extern "C" int lmgx_set_dirichlet(lmgx_handle *h, const int *flags)
             h->dirich[b][s] = flags ? (flags[6 * b + s] != 0 && !is_swap(h->bc[b][s])) : 0;
             for (int l = 0; l < h->nlev; ++l) h->lev[l].h_lb[b].dir[s] = h->dirich[b][s];
         }
-    for (int l = 0; l < h->nlev; ++l)
-        HIPCHK(hipMemcpy(h->lev[l].d_lb, h->lev[l].h_lb.data(), sizeof(LevBlk) * h->nblk,
-                         hipMemcpyHostToDevice));
+    for (int l = 0; l < h->nlev; ++l) CHK(copy_tables(h, l));
     return 0;
 }
 
