@@ -56,6 +56,33 @@ def test_fast_converges_to_direct_solution(cfg, scale, prec):
     assert np.all(np.abs(r - (bh[m] - A @ x)) <= 1e-13 * scale)
 
 
+def test_sixteen_blocks_residual_and_precisions():
+    """The benchmark's 16-block topology (c3 at 1/8 scale: 233,472 cells, 7 x-chains with a
+    face-Dirichlet outflow each, periodic z): both precisions reach a 1e-11 reduction of the
+    residual of scipy's assembled operator (b - A f from the returned f), return that residual,
+    and agree on the pressure to 1e-9 relative."""
+    import torch
+    p, o, s, coeffs, b = setup("c3", 8)
+    A, idx = assemble_level1(o, coeffs)
+    m = interior_mask(o, 0)
+    bh = b.cpu().numpy()[: o.n1][m]
+    xs = []
+    for prec in (0, 1):
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b, res, max_cycles=60, tolf=0.0, rtol=1e-11, smoother=1,
+                           precision=prec, numpre=1, numpost=1, omega=1.35, omega_c=1.0,
+                           sweeps_per_iter=3)
+        assert len(hist) < 60
+        x = f.cpu().numpy()[: o.n1][m]
+        r = bh - A @ x
+        assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(bh)
+        rg = res.cpu().numpy()[: o.n1][m]
+        assert np.all(np.abs(rg - r) <= 1e-13 * (np.abs(bh) + abs(A) @ np.abs(x)))
+        xs.append(x)
+    assert np.linalg.norm(xs[1] - xs[0]) <= 1e-9 * np.linalg.norm(xs[0])
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_new_operator_between_solves(prec):
     """A second operator on the same handle (here 2A) must not reuse the first one's captured
