@@ -59,7 +59,12 @@ def parse():
                     help="also time the same solve with the level work launched block by block "
                          "(a per-block launch loop) instead of once per level for all blocks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cycles", type=int, default=3)
+    ap.add_argument("--cpu-cycles", type=int, default=3,
+                    help="cycles of the single-chain CPU baseline (the fallback)")
+    ap.add_argument("--cpu-cycles-all", type=int, default=1,
+                    help="cycles of the whole-workload CPU baseline (all x-chains at once)")
+    ap.add_argument("--cpu-one-chain", action="store_true",
+                    help="CPU baseline on the first x-chain only, one core")
     ap.add_argument("--no-stream-ceiling", action="store_true",
                     help="skip the measured streaming rate of the box (torch kernels)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -255,6 +260,54 @@ def timed(s, b, args, mode, steps, warmup):
     return dt, cycles, f, hist
 
 
+def x_chains(p):
+    """The problem's x-chains (blocks linked east-west), each independent of the others."""
+    chains = []
+    for i, blk in enumerate(p.blocks):
+        if blk.bl2bl[0] == 0:
+            c = [i]
+            while p.blocks[c[-1]].bl2bl[1]:
+                c.append(p.blocks[c[-1]].bl2bl[1] - 1)
+            chains.append(c)
+    return chains
+
+
+def cpu_baseline_all_chains(args):
+    """The reference Fortran (oracle/_ref/ref_driver) on the whole workload: every x-chain in its
+    own single-rank process, all at once, one core each.  The chains share no halo, so this is
+    the reference's MPI decomposition by chain, without the global norm reductions.  Each
+    process runs the reference's own coeff_p on the chain's grid (density 1, no face-Dirichlet
+    outflow: the same sweeps per cycle as the product's operator) and a seeded source, FP64
+    V(2,2) point SOR, ``--cpu-cycles`` cycles (1 by default here).  Throughput = all cells x
+    cycles / the slowest process's solve time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from fortranutils_amd import problems as P
+    import ref_io
+    if not ref_io.available():
+        raise RuntimeError("reference driver not built")
+    p = P.config(args.config, args.scale)
+    chains = x_chains(p)
+    subs = [P.subproblem(p, c) for c in chains]
+    rng = np.random.default_rng(11)
+    b1s = []
+    for q in subs:
+        n1 = sum(int(np.prod(b.nijkp1)) for b in q.blocks)
+        b1s.append(rng.standard_normal(n1))
+    ncyc = args.cpu_cycles_all
+    out = ref_io.run_timing_concurrent(subs, ncycles=ncyc, b1s=b1s)
+    del b1s
+    cells = [q.interior_cells() for q in subs]
+    tmax = max(t for t, _ in out)
+    big = max(cells) / sum(cells)
+    sample = (f"{args.config} whole: {p.nblk} blocks, {sum(cells) / 1e6:.1f}M cells, as its "
+              f"{len(chains)} independent x-chains in {len(chains)} concurrent single-rank "
+              f"reference processes (largest chain {100 * big:.0f}% of the cells); the "
+              f"reference's own coeff_p, FP64 V(2,2) point SOR, {ncyc} cycle(s)")
+    return dict(value=sum(cells) * ncyc / tmax / 1e6, unit="Mcell*V-cycles/s", cores=len(chains),
+                ranks=len(chains), kind="reference", sample=sample, seconds=tmax,
+                per_process=[dict(cells=c, seconds=t) for c, (t, _) in zip(cells, out)])
+
+
 def cpu_baseline(args):
     """The reference Fortran (oracle/_ref/ref_driver, built from the reference sources) when it
     runs here, else the C restatement: FP64 V(2,2) with the reference's point-SOR schedule on the
@@ -285,8 +338,8 @@ def cpu_baseline(args):
             out = ref_io.run(q, ncycles=args.cpu_cycles, coeffs=a, b1=bb[: o.n1], n1=o.n1,
                              do_post=0, timeout=600)
             return dict(value=cells * args.cpu_cycles / out["tsolve"] / 1e6,
-                        unit="Mcell*V-cycles/s", cores=1, kind="reference", sample=sample,
-                        seconds=out["tsolve"])
+                        unit="Mcell*V-cycles/s", cores=1, ranks=1, kind="reference",
+                        sample=sample, seconds=out["tsolve"])
         except Exception as e:  # the reference build may not run on this host
             print("reference CPU baseline unavailable:", str(e)[:200], file=sys.stderr)
     f = np.zeros(o.ntot)
@@ -294,7 +347,7 @@ def cpu_baseline(args):
     o.solve(a, f, bb, args.cpu_cycles)
     dt = time.perf_counter() - t0
     return dict(value=cells * args.cpu_cycles / dt / 1e6, unit="Mcell*V-cycles/s", cores=1,
-                kind="port", sample=sample, seconds=dt)
+                ranks=1, kind="port", sample=sample, seconds=dt)
 
 
 def traffic_from_profiles(workload, kernel):
@@ -515,10 +568,19 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        try:
-            cpu = cpu_baseline(args)
-        except Exception as e:
-            print("cpu baseline failed:", e, file=sys.stderr)
+        if not args.cpu_one_chain:
+            try:
+                cpu = cpu_baseline_all_chains(args)
+            except Exception as e:
+                print("whole-workload CPU baseline unavailable:", str(e)[:300], file=sys.stderr)
+        if cpu is None:
+            try:
+                cpu = cpu_baseline(args)
+            except Exception as e:
+                print("cpu baseline failed:", e, file=sys.stderr)
+        if cpu:
+            print(f"# CPU baseline ({cpu['kind']}, {cpu['cores']} cores): {cpu['value']:.2f} "
+                  f"{cpu['unit']}", file=sys.stderr, flush=True)
 
     # the whole solve against the HBM roofline: algorithmic bytes of the schedule / time
     sb = solve_bytes(s, args, args.mode, cycles / args.steps) if args.mode != "exact" else None

@@ -23,7 +23,8 @@ C   per block: r8 x(2:nip1,2:njp1), y(2:nip1,2:njp1), z(2:nkp1)
 C   coeff_src=1: r8 ap,aw,ae,as,an,al,ar  (ntot each, mg layout)
 C   src_src=0  : r8 u,v,w,rho,rhonp1 (n1 each)   src_src=1: r8 b (n1)
 C   r8  f (n1)   initial guess on level 1
-C Output: see the write statements at the end.
+C Output: see the write statements at the end; do_post < 0 writes only
+C   i4 4713, ncycles, nummgl; r8 tsolve; r8 hist(4,nummgl,ncycles).
 C=======================================================================
 
       program ref_driver
@@ -233,6 +234,19 @@ C SO ONLY THE FIRST REPEAT IS KEPT FOR PARITY (ntime=1 FOR GOLDENS).
         t1 = MPI_WTIME()
         tsolve = tsolve + (t1 - t0)
       end do
+
+C TIMING ONLY (do_post < 0): THE SOLVE TIME AND THE NORMS, NO FIELDS
+
+      if (do_post .lt. 0) then
+        open(unit=11, file=fout, access='stream', form='unformatted',
+     &       status='replace')
+        write(11) 4713, ncycles, nummgl
+        write(11) tsolve
+        write(11) hist
+        close(11)
+        call MPI_FINALIZE(ierr)
+        stop
+      end if
 
       open(unit=11, file=fout, access='stream', form='unformatted',
      &     status='replace')

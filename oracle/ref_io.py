@@ -132,6 +132,38 @@ def run(problem, *, ncycles: int, coeffs=None, b1=None, fields=None, f0=None, n1
         return out
 
 
+def run_timing_concurrent(problems, *, ncycles: int, b1s, timeout: float = 900.0):
+    """Time the reference's lmg_solve on independent problems at once, one single-rank driver
+    process per problem (the reference's own coeff_p on each problem's grid, the given level-1
+    sources, zero initial guess).  Returns [(tsolve seconds, hist[ncycles, nummgl, 4])] in
+    order.  Independent problems in concurrent processes do what an MPI run with one rank per
+    problem does, minus the global norm reductions."""
+    if not available():
+        raise RuntimeError("reference driver not built: make -C oracle ref")
+    with tempfile.TemporaryDirectory() as td:
+        procs = []
+        for q, (prob, b1) in enumerate(zip(problems, b1s)):
+            fin, fout = os.path.join(td, f"in{q}.bin"), os.path.join(td, f"out{q}.bin")
+            write_input(fin, prob, ncycles=ncycles, b1=b1, n1=b1.size, do_post=-1)
+            log = open(os.path.join(td, f"log{q}.txt"), "wb")     # no pipe can stall a run
+            procs.append((subprocess.Popen([DRIVER, fin, fout], stdout=log,
+                                           stderr=subprocess.STDOUT, cwd=td), fout, log))
+        out = []
+        for pr, fout, log in procs:
+            pr.wait(timeout=timeout)
+            log.close()
+            if pr.returncode != 0 or not os.path.exists(fout):
+                msg = open(log.name, "rb").read()[-400:]
+                raise RuntimeError(f"ref_driver failed ({pr.returncode}): {msg!r}")
+            raw = open(fout, "rb").read()
+            magic, ncyc, L = np.frombuffer(raw, "<i4", 3, 0)
+            assert magic == 4713, "bad timing output magic"
+            t = float(np.frombuffer(raw, "<f8", 1, 12)[0])
+            hist = np.frombuffer(raw, "<f8", 4 * L * ncyc, 20).reshape(ncyc, L, 4).copy()
+            out.append((t, hist))
+        return out
+
+
 def level1_fields(problem, fields_per_block: dict) -> dict:
     """Concatenate per-block [k,j,i] arrays into the level-1 mg layout (blocks in order)."""
     return {k: np.concatenate([a.ravel() for a in v]) for k, v in fields_per_block.items()}
