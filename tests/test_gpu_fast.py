@@ -83,6 +83,32 @@ def test_sixteen_blocks_residual_and_precisions():
     assert np.linalg.norm(xs[1] - xs[0]) <= 1e-9 * np.linalg.norm(xs[0])
 
 
+def test_coarsest_direct_solve_by_components():
+    """The coarsest level of c3/8 has 7 independent x-chains: the direct solve factors 7 dense
+    blocks at once.  With it the V-cycle reaches a 1e-11 reduction in at most 12 cycles, and the
+    pressure has the residual of scipy's assembled operator; relaxing the coarsest level instead
+    (the reference's solversc) stalls on these chains, taking at least three times as many."""
+    import torch
+    p, o, s, coeffs, b = setup("c3", 8)
+    A, idx = assemble_level1(o, coeffs)
+    m = interior_mask(o, 0)
+    bh = b.cpu().numpy()[: o.n1][m]
+    cyc = {}
+    for cs in (1, 0):
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b, res, max_cycles=60, tolf=0.0, rtol=1e-11, smoother=1,
+                           precision=0, numpre=1, numpost=1, omega=1.35, omega_c=1.0,
+                           sweeps_per_iter=3, coarse_solve=cs)
+        cyc[cs] = len(hist)
+        if cs == 1:
+            x = f.cpu().numpy()[: o.n1][m]
+            assert np.linalg.norm(bh - A @ x) <= 1e-10 * np.linalg.norm(bh)
+    print("cycles direct / relaxed", cyc[1], cyc[0])
+    assert cyc[1] <= 12
+    assert cyc[0] >= 3 * cyc[1]
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_new_operator_between_solves(prec):
     """A second operator on the same handle (here 2A) must not reuse the first one's captured
