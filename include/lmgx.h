@@ -103,7 +103,9 @@ int lmgx_set_operator_device(lmgx_handle *h, const double *const d_coeff[7]);
  * swap_sc0.F, init_array.F.  f: initial guess in (level 1), solution out (all levels); b: source
  * in (level 1), coarse right-hand sides out; residu: residuals out (all levels).  Host arrays of
  * ntot doubles.  history (may be NULL): [max_cycles][nlev][4] = (rnorm, fnorm, enorm, vnorm) after
- * each cycle, as in /cbmgdata2/ (cb.control.inc:156-164); ncycles_out: cycles run. */
+ * each cycle, as in /cbmgdata2/ (cb.control.inc:156-164); the fast path (smoother =
+ * LMGX_SMOOTH_LINE) measures level 1 only and reports zeros for the coarser levels;
+ * ncycles_out: cycles run. */
 int lmgx_solve(lmgx_handle *h, double *f, double *b, double *residu, const lmgx_params *prm,
                double *history, int *ncycles_out);
 /* Same on caller-owned device arrays (ntot doubles each, stream-ordered on the handle's stream,

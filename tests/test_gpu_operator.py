@@ -76,3 +76,47 @@ def test_update_density_matches_build(cfg, scale):
         out.append((f.cpu().numpy(), hist))
     assert np.array_equal(out[0][0], out[1][0])
     assert len(out[0][1]) < 30                  # reached 1e-10 before the cycle cap
+
+
+@pytest.mark.parametrize("first", [0, 1])
+@pytest.mark.parametrize("cfg,scale", [("c4", 8), ("c3", 8)])
+def test_update_density_after_a_solve(cfg, scale, first, monkeypatch):
+    """Once a fast solve has run, lmgx_update_density_device writes level 1 of the fast path's
+    split operator itself (FP64, and FP32 when the FP32 operator exists) and takes the symmetry
+    of its operator as known.  A mixed solve after the update must then equal, bitwise, the same
+    solve on a second solver given the same coefficients through lmgx_set_operator (split copies
+    made from the natural layout, symmetry checked).  first: precision of the solve before the
+    update (0: only the FP64 split exists then, 1: both)."""
+    import torch
+    from fortranutils_amd import lmgx
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg")     # the same kernels on both solvers
+    p = P.config(cfg, scale)
+    o = O.Oracle(p)
+    geo = o.geometry()
+    fl = fields(p, o)
+    dirich = np.array([b.dirichlet for b in p.blocks], np.int32)
+    rng = np.random.default_rng(5)
+    rho2 = fl["rho"] * (1.0 + 0.3 * rng.random(o.n1))
+    bsrc = torch.from_numpy(o.sour(geo, fl["u"], fl["v"], fl["w"], fl["rho"], rho2, p.dtime)).cuda()
+    kw = dict(smoother=1, rtol=0.0, tolf=0.0, max_cycles=4, numpre=1, numpost=1,
+              sweeps_per_iter=3, omega=1.35)
+
+    def solve(s, prec):
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, bsrc.clone(), res, precision=prec, **kw)
+        return f.cpu().numpy(), hist
+
+    a = lmgx.Solver(lmgx.Topology.from_problem(p))
+    a.build_operator(p, rho1=fl["rho"], dirichlet=dirich)
+    solve(a, first)
+    a.update_density(torch.from_numpy(rho2).cuda())
+    fa, ha = solve(a, 1)
+    c = lmgx.Solver(lmgx.Topology.from_problem(p))
+    c.build_operator(p, rho1=fl["rho"], dirichlet=dirich)
+    solve(c, first)
+    c.set_operator([torch.from_numpy(x).cuda() for x in a.get_operator()])
+    fc, hc = solve(c, 1)
+    assert np.array_equal(fa, fc)
+    assert np.array_equal(ha, hc)
+
