@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--cpu-one-chain", action="store_true",
                     help="CPU baseline on the first x-chain only, one core")
     ap.add_argument("--no-stream-ceiling", action="store_true",
-                    help="skip the measured streaming rate of the box (torch kernels)")
+                    help="skip the measured streaming rate of this GPU (torch kernels)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprof FETCH/WRITE_SIZE)")
     ap.add_argument("--pmc-summary", nargs="+", metavar="CSV")

@@ -1,6 +1,6 @@
 /* TEST INFRASTRUCTURE ONLY - plain-C restatement of the reference LESter pressure-Poisson path
  * (code/lmg_solve.F and everything it calls, plus coeff_p/sour_p/extrap_p/p_ref_subtract and the
- * geometry that feeds coeff_p).  It is the checker for the HIP library: only tests/,
+ * geometry that feeds coeff_p).  It is the oracle for the HIP library: only tests/,
  * __graft_entry__.smoke() and bench.py's CPU-baseline leg load it, and never as the measured or
  * shipped path.
  *

@@ -1,5 +1,5 @@
 /* TEST INFRASTRUCTURE ONLY - CPU restatement of the reference LESter pressure path.
- * Used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as the checker; the
+ * Used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as the oracle; the
  * product library (fortranutils_amd/csrc) never links or calls it.
  *
  * Arrays use the reference's multi-level layout: level l (0-based here), block b starts at

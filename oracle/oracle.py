@@ -1,6 +1,6 @@
 """TEST INFRASTRUCTURE ONLY: ctypes wrapper of ``oracle/liblmg_oracle.so``, the plain-C
 restatement of the reference pressure path.  Only ``tests/``, ``__graft_entry__.smoke()`` and
-``bench.py``'s CPU-baseline leg import it, as the checker.
+``bench.py``'s CPU-baseline leg import it, as the oracle the library is checked against.
 """
 from __future__ import annotations
 

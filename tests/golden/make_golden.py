@@ -13,7 +13,7 @@ injected, those arrays) and the reference's outputs.  Cases:
   g4_c0s4_f    as g1 but F-cycles, 4 cycles
   g5_c4s8_ext  2 blocks with 1/rho face scaling and face-Dirichlet outflow (coefficients and
                mean-free source injected), 8 V(2,2)
-  g6_c0        configs[0] at full size (64x64x32, 4 levels, Dirichlet outflow, mean-free
+  g6_c0        c0 at full size (64x64x32, 4 levels, Dirichlet outflow, mean-free
                divergence source), 30 V(2,2): norm history, p after extrap_p + p_ref_subtract
                (strided sample + checksums)
   g7_c4s8_uvw  as g2 with rho^{n+1} = rho (1 + 10% seeded noise), 3 V(2,2), then the velocity

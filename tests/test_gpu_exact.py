@@ -61,7 +61,7 @@ def test_device_entry_point_and_post_steps(name):
 
 
 def test_c0_full_30_cycles():
-    """64x64x32, 4 levels, V(2,2), 30 cycles; history to 1e-12, p bit-exact."""
+    """c0: one 64x64x32 block, 4 levels, V(2,2), 30 cycles; history to 1e-12, p bit-exact."""
     import torch
     p, o, coeffs, b, g = case("g6_c0")
     s = solver_for(p)

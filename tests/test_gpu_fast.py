@@ -88,8 +88,8 @@ def test_sixteen_blocks_residual_and_precisions():
 @functools.lru_cache(maxsize=None)
 def _reference_converged(cfg, scale):
     """The reference Fortran (oracle/_ref/ref_driver) run long enough to converge to rounding
-    (it contracts about 0.6 per cycle at the end on these single-chain problems): configs[0] at
-    full size with the inputs of golden g6, or the oracle's operator (1/rho_face scaling, face-
+    (it contracts about 0.6 per cycle at the end on these single-chain problems): c0 at full
+    size with the inputs of golden g6, or the oracle's operator (1/rho_face scaling, face-
     Dirichlet outflow) and source at the given scale.  Returns (problem, oracle, operator, b,
     p after extrap_p and p_ref_subtract, the reference's residual history)."""
     import ref_io
@@ -121,8 +121,9 @@ def test_pressure_matches_reference_fortran(cfg, scale, prec):
     """The fast path's pressure, FP64 and mixed, solved to a 1e-12 reduction and followed by
     extrap_p and p_ref_subtract, against the pressure of the reference Fortran's own lmg_solve
     converged to rounding on the same operator and source: 1e-9 relative L2 over level 1.
-    configs[0] at full size (64x64x32, density 1), configs[2] at 1/8 (the 1:7 mixing layer on a
-    stretched single block) and configs[4] at 1/8 (two blocks across an interface).  (The
+    c0 at full size (one 64x64x32 block, density 1), c2 at 1/8 (the 1:7 mixing layer on a
+    stretched 64x32x32 block) and c4 at 1/8 (two blocks, 16x12x8 and 12x12x8, across an
+    interface).  (The
     reference's 30-cycle pressure of golden g6 is not converged that far: its last cycles
     contract about 0.6 each, and it differs from the converged one by ~3e-7.)"""
     import torch

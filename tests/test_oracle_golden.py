@@ -47,7 +47,7 @@ def test_solve_bitwise(name):
 
 
 def test_c0_full_history_and_pressure():
-    """configs[0] at full size: 30 V(2,2) cycles, 4 levels, Dirichlet outflow."""
+    """c0 (one 64x64x32 block) at full size: 30 V(2,2) cycles, 4 levels, Dirichlet outflow."""
     p, o, coeffs, b, g = case("g6_c0")
     f = np.zeros(o.ntot)
     hist, res = o.solve(coeffs, f, b, 30)

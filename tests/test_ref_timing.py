@@ -13,7 +13,7 @@ pytestmark = pytest.mark.skipif(not ref_io.available(),
 
 def test_timing_mode_runs_the_same_solve():
     """Two x-chains of c3 at 1/8, timed concurrently: each history equals, bitwise, the history
-    of the driver's full-output run of the same chain (the reference's own coeff_p, the same
+    of the reference driver's full-output run of the same chain (the reference's own coeff_p, the same
     seeded source), and the solve times are positive."""
     p = P.config("c3", 8)
     chains = [[0, 1], [2, 3, 4]]
