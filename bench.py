@@ -549,7 +549,9 @@ def main():
         tm = int(s.tile_widths()[1 if word == 4 else 0][0])
         rl = "true" if args.omega != 1.0 else "false"
         fp = "float" if word == 4 else "double"
-        key = (f"k_zebra_seg<{fp}, 1, {rl}, 0>" if tm == 16 else f"k_zebra_line<{fp}, 1, {tm}, {rl}>")
+        fz = "true" if s.fused_halos()[1 if word == 4 else 0][0] else "false"
+        key = (f"k_zebra_seg<{fp}, 1, {rl}, 0>" if tm == 16
+               else f"k_zebra_line<{fp}, 1, {tm}, {rl}, {fz}>")
         tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=achieved / PEAK_HBM, traffic=tr,

@@ -207,6 +207,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             const int *n = h->nijkp1[l][b];
             LevBlk &B = D.h_lb[b];
             memset(&B, 0, sizeof(B));
+            for (int q = 0; q < 4; ++q) B.fz_off[q] = -1;
             B.off = h->off[l][b];
             B.n[0] = n[0]; B.n[1] = n[1]; B.n[2] = n[2];
             // row pitch of the split layout: ceil(nip1/2) slots rounded up to 32, so every row

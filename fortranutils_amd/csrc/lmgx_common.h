@@ -43,6 +43,13 @@ struct LevBlk {
     long o1;         // offset of the block's 1-D k metric arrays
     long oxy;        // offset of the block's vertex array (nip1+1)*(njp1+1)
     long oz;         // offset of the block's k-vertex array (nkp1+1)
+    // Halo stores made by the segmented line solve itself.  Side q = 0..3 is the interior plane
+    // i = 1, i = nip1-2, k = 1, k = nkp1-2; when it is the source of a partner's face-interior
+    // halo, fz_off[q] >= 0 is that block's off_s and the halo point of source point (I,J,K) is
+    // (I,J,K) + fz_d[q] in a block of row pitch fz_hp[q] and fz_n1[q] rows; -1 = no store.
+    long fz_off[4];
+    int fz_hp[4], fz_n1[4];
+    int fz_d[4][3];
 };
 
 // One swapped face of one block on one level (reference swap_sc0 tables).

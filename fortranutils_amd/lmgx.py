@@ -311,6 +311,12 @@ class Solver:
         _check(lib().lmgx_tile_widths(self.h, _ip(out)))
         return out.reshape(2, MAXL)[:, : self.nlev]
 
+    def fused_halos(self):
+        """[precision][level]: 1 where the line kernel stores the face-interior halos itself."""
+        out = np.zeros(2 * MAXL, np.int32)
+        _check(lib().lmgx_fused_halos(self.h, _ip(out)))
+        return out.reshape(2, MAXL)[:, : self.nlev]
+
     # ---- profiling ----------------------------------------------------------------------
     def profile(self, enable=True):
         _check(lib().lmgx_profile(self.h, int(enable)))

@@ -225,6 +225,15 @@ int lmgx_rho_adjoint_device(lmgx_handle *h, const double *d_lambda, const double
  * FP64, 1 = FP32; 0 for levels that do not exist or before the first fast solve. */
 int lmgx_tile_widths(const lmgx_handle *h, int *out);
 
+/* Levels whose face-interior halos the fast path's line kernels store themselves instead of a
+ * swap launch after each half-sweep: out[precision][LMGX_MAXL] like lmgx_tile_widths, 1 = stored
+ * by the line kernel, 0 = swapped.  A level qualifies when every swapped face is an i- or k-face
+ * copying a whole plane of its partner with the same in-plane axes and the same colour; the
+ * autotuner then times each line kernel both ways (a pinned kernel stores them).  All 0 before
+ * the first fast solve of a precision, or with LMGX_NO_FUSED_SWAP=1 in the environment at the
+ * first fast solve. */
+int lmgx_fused_halos(const lmgx_handle *h, int *out);
+
 /* Kernel timing of the last solve: total duration (ms), count and updated cells of the dominant
  * smoother kernel, measured with HIP events on the handle's stream.  The fast path times the
  * finest level's plain half-sweeps (not the two zero-start half-sweeps that open each cycle's

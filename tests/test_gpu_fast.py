@@ -241,6 +241,37 @@ def test_per_block_launches_agree(cfg, scale, prec, monkeypatch):
     assert np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("prec,kernel", [(1, "seg"), (1, "128"), (0, "64")])
+@pytest.mark.parametrize("cfg,scale", [("c1", 2), ("c3", 4), ("c0", 1)])
+def test_fused_halo_stores_match_swaps(cfg, scale, prec, kernel, monkeypatch):
+    """The line kernels (segmented, two-ended) storing the i- and k-face halos themselves give
+    bitwise the pressure, residual and history of the same solve with a swap launch after every
+    half-sweep (LMGX_NO_FUSED_SWAP=1), in mixed precision and in FP64.  c1 and c3 have
+    x-interfaces and z-periodic sides, c0 z-periodic sides only."""
+    import torch
+    monkeypatch.setenv("LMGX_LINE_KERNEL", kernel)
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("LMGX_NO_FUSED_SWAP", mode)
+        p, o, s, coeffs, b = setup(cfg, scale)
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b, res, max_cycles=4, tolf=0.0, rtol=0.0, smoother=1, precision=prec,
+                           numpre=1, numpost=1, omega=1.35, omega_c=1.0, sweeps_per_iter=3)
+        fz = s.fused_halos()[prec]
+        tw = s.tile_widths()
+        print(cfg, prec, kernel, "fused levels", fz.tolist(), "widths", tw.tolist())
+        # every level above the coarsest qualifies (the coarsest may have odd extents, where a
+        # source point and its halo point have different colours); the pinned kernel runs on
+        # every level it is valid for
+        assert (fz[:-1] == (0 if mode == "1" else 1)).all()
+        assert mode == "0" or not fz.any()
+        assert (tw[prec][:-1] == (16 if kernel == "seg" else int(kernel))).all()
+        out.append((f.cpu().numpy(), hist, res.cpu().numpy()))
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_nonsymmetric_operator(prec):
     """A non-symmetric operator (every coupling perturbed by up to 1%) is detected: the residual
