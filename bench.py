@@ -123,8 +123,10 @@ def stream_ceiling(reps=5):
 def solve_bytes(s, args, mode, cycles):
     """Algorithmic HBM bytes of one solve of ``cycles`` cycles with this schedule, per kernel and
     level (halo swaps, O(surface), count 0; symmetric operator, as the solver checks):
-      half-sweep: 10 words per updated cell (9 with omega = 1); the two zero-start half-sweeps
-        that open a level's correction on the segmented kernel read 8 and 9 words;
+      half-sweep: 10 words per updated cell (9 with omega = 1); on a level whose FP32 segmented
+        solve reads binary16 coefficients the 6 operator words are 2 bytes each (28 / 24 bytes);
+        the two zero-start half-sweeps that open a level's correction on the segmented kernel
+        read 2 and 1 fewer words of x;
       zeroing a level's correction where no zero start applies: 1 word per cell;
       restriction: residual from b, x, ap, ae, an, ar (6 words per fine cell) + the coarse b;
       prolongation + correction: fine x read and written + coarse x;
@@ -139,10 +141,12 @@ def solve_bytes(s, args, mode, cycles):
     w = [4 if mixed and l < L - 1 else 8 for l in range(L)]
     hw = 10 if args.omega != 1.0 else 9
     tw = s.tile_widths()[1 if mixed else 0]
+    c16 = s.coef16_levels() if mixed else np.zeros(L, np.int32)
     npre = args.numpre * args.spi
     cyc = 0.0
     for l in range(L - 1):
-        cyc += (args.numpre + args.numpost) * args.spi * N[l] * hw * w[l]
+        hb = hw * w[l] - (6 * w[l] - 12 if c16[l] and int(tw[l]) == 16 else 0)
+        cyc += (args.numpre + args.numpost) * args.spi * N[l] * hb
         if mixed and npre >= 1 and int(tw[l]) == 16:
             cyc -= 0.5 * N[l] * ((hw - 8) + (hw - 9)) * w[l]     # zero-start half-sweeps
         elif mixed or l > 0:
@@ -537,26 +541,30 @@ def main():
     # stream.  Algorithmic bytes per updated cell, whatever the kernel: the right-hand side b,
     # 6 operator words (ap, aw, ae, al, ar and the line coupling as; an is the next row's as),
     # 1 word of the other colour's x, 1 word written (x) - 9 words; 10 when over-relaxed (the
-    # old x is read).
+    # old x is read).  With binary16 coefficients (mixed precision) the 6 operator words are 2
+    # bytes each (ap replaced by the diagonal excess d): 28 bytes over-relaxed, 24 without.
     roof = None
     if args.mode != "exact":
         _, _, (ms, nl, cells) = solve(s, b, args, args.mode, args.rtol, profile=True)
         word = 4 if args.mode == "mixed" else 8
         words = 9 if args.omega == 1.0 else 10
-        per_launch_bytes = (cells / max(nl, 1)) * words * word
+        tm = int(s.tile_widths()[1 if word == 4 else 0][0])
+        h16 = word == 4 and tm == 16 and bool(s.coef16_levels()[0])
+        bytes_per_cell = words * word - (6 * word - 12 if h16 else 0)
+        per_launch_bytes = (cells / max(nl, 1)) * bytes_per_cell
         avg_s = ms / max(nl, 1) / 1e3
         achieved = per_launch_bytes / avg_s
-        tm = int(s.tile_widths()[1 if word == 4 else 0][0])
         rl = "true" if args.omega != 1.0 else "false"
         fp = "float" if word == 4 else "double"
         fz = "true" if s.fused_halos()[1 if word == 4 else 0][0] else "false"
-        key = (f"k_zebra_seg<{fp}, 1, {rl}, 0>" if tm == 16
+        key = (f"k_zebra_seg<{fp}, 1, {rl}, 0, {'true' if h16 else 'false'}>" if tm == 16
                else f"k_zebra_line<{fp}, 1, {tm}, {rl}, {fz}>")
         tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=achieved / PEAK_HBM, traffic=tr,
                     kernel=key, avg_launch_us=1e6 * avg_s, launches=nl,
-                    bytes_per_launch=per_launch_bytes, words_per_cell=words,
+                    bytes_per_launch=per_launch_bytes, bytes_per_cell=bytes_per_cell,
+                    coef16=h16, gcell_updates_per_s=cells / max(nl, 1) / avg_s / 1e9,
                     cells_per_launch=cells / max(nl, 1))
         print(f"# smoother: {nl} launches, avg {1e6 * avg_s:.1f} us, {achieved / 1e9:.0f} GB/s, "
               f"tile widths fp64/fp32 {s.tile_widths().tolist()}", file=sys.stderr, flush=True)

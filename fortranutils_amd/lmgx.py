@@ -317,6 +317,12 @@ class Solver:
         _check(lib().lmgx_fused_halos(self.h, _ip(out)))
         return out.reshape(2, MAXL)[:, : self.nlev]
 
+    def coef16_levels(self):
+        """Per level: 1 where the FP32 line relaxation reads binary16 coefficients."""
+        out = np.zeros(MAXL, np.int32)
+        _check(lib().lmgx_coef16_levels(self.h, _ip(out)))
+        return out[: self.nlev]
+
     # ---- profiling ----------------------------------------------------------------------
     def profile(self, enable=True):
         _check(lib().lmgx_profile(self.h, int(enable)))

@@ -234,6 +234,16 @@ int lmgx_tile_widths(const lmgx_handle *h, int *out);
  * first fast solve. */
 int lmgx_fused_halos(const lmgx_handle *h, int *out);
 
+/* Levels whose FP32 segmented line relaxation (mixed precision) reads binary16 copies of the
+ * couplings, scaled by a power of two per level, and forms the diagonal from them plus the
+ * level's Dirichlet excess ap - sum(a_nb): out[LMGX_MAXL], 1 = binary16, 0 = FP32.  A level
+ * qualifies when its couplings are non-negative and, with the diagonal excess, span at most 2^28.
+ * The relaxation then works on an operator whose edge weights are within 2^-11 of the true ones
+ * (symmetric, zero row sums kept), which the FP64 refinement absorbs.  All 0 before the first
+ * mixed solve or with LMGX_COEF16=0 in the environment at the first mixed solve after an operator
+ * is set. */
+int lmgx_coef16_levels(const lmgx_handle *h, int *out);
+
 /* Kernel timing of the last solve: total duration (ms), count and updated cells of the dominant
  * smoother kernel, measured with HIP events on the handle's stream.  The fast path times the
  * finest level's plain half-sweeps (not the two zero-start half-sweeps that open each cycle's
