@@ -128,7 +128,8 @@ def solve_bytes(s, args, mode, cycles):
         the two zero-start half-sweeps that open a level's correction on the segmented kernel
         read 2 and 1 fewer words of x;
       zeroing a level's correction where no zero start applies: 1 word per cell;
-      restriction: residual from b, x, ap, ae, an, ar (6 words per fine cell) + the coarse b;
+      restriction: residual from b, x, ap, ae, an, ar (6 words per fine cell; binary16 d, ae,
+        an, ar where the level's line solve reads those) + the coarse b;
       prolongation + correction: fine x read and written + coarse x;
       coarsest direct solve: the dense inverse (n^2 FP64) and two vectors;
       level 1, mixed: FP64 residual + norms (6 FP64 words) writing the FP32 right-hand side, and
@@ -151,7 +152,8 @@ def solve_bytes(s, args, mode, cycles):
             cyc -= 0.5 * N[l] * ((hw - 8) + (hw - 9)) * w[l]     # zero-start half-sweeps
         elif mixed or l > 0:
             cyc += N[l] * w[l]                                     # zeroed before use
-        cyc += 6 * N[l] * w[l] + N[l + 1] * w[l + 1]               # residual + restriction
+        h16 = bool(c16[l]) and int(tw[l]) == 16
+        cyc += (2 * w[l] + 8 if h16 else 6 * w[l]) * N[l] + N[l + 1] * w[l + 1]   # restriction
         cyc += 2 * N[l] * w[l] + N[l + 1] * w[l + 1]               # prolongation + correction
     nc = N[L - 1]
     cyc += 8 * (nc * nc + 2 * nc)                                  # coarsest direct solve
