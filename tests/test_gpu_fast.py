@@ -298,6 +298,43 @@ def test_coef16_relaxation_converges_like_fp32(cfg, scale, monkeypatch):
     assert np.linalg.norm(x16 - x32) <= 1e-9 * np.linalg.norm(x32)
 
 
+def test_coef16_falls_back_on_a_wide_coefficient_range(monkeypatch):
+    """A density of 1e7 on the upper half of every block (in z) spreads the couplings of every
+    level over more than 2^28, so binary16 would lose the small ones: the levels keep FP32
+    couplings (reported 0), and the mixed solve still reaches the FP64 solve's pressure."""
+    import torch
+    from fortranutils_amd import lmgx
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg")
+    p = P.config("c4", 8)
+    o = O.Oracle(p)
+    fl = fields(p, o)
+    dirich = np.array([b.dirichlet for b in p.blocks], np.int32)
+    rho = np.ones(o.n1)
+    for b in range(o.nblk):
+        n = o.nijkp1[0, b]
+        v = rho[o.off[0][b]: o.off[0][b] + n[0] * n[1] * n[2]].reshape(n[2], n[1], n[0])
+        v[n[2] // 2:] = 1e7
+    out = []
+    for prec in (1, 0):
+        s = lmgx.Solver(lmgx.Topology.from_problem(p))
+        s.build_operator(p, rho1=rho, dirichlet=dirich)
+        b = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        dev = lambda a: torch.from_numpy(a).cuda()
+        s.source(dev(fl["u"]), dev(fl["v"]), dev(fl["w"]), dev(fl["rho"]), dev(fl["rhonp1"]),
+                 p.dtime, b)
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b, res, max_cycles=200, tolf=0.0, rtol=1e-10, smoother=1,
+                           precision=prec, numpre=1, numpost=1, omega=1.35, omega_c=1.0,
+                           sweeps_per_iter=3)
+        print("precision", prec, "cycles", len(hist), "coef16", s.coef16_levels().tolist())
+        assert len(hist) < 200
+        if prec == 1:
+            assert not s.coef16_levels().any()
+        out.append(f.cpu().numpy()[: o.n1][interior_mask(o, 0)])
+    assert np.linalg.norm(out[0] - out[1]) <= 1e-8 * np.linalg.norm(out[1])
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_nonsymmetric_operator(prec):
     """A non-symmetric operator (every coupling perturbed by up to 1%) is detected: the residual
