@@ -146,13 +146,13 @@ def solve_bytes(s, args, mode, cycles):
     npre = args.numpre * args.spi
     cyc = 0.0
     for l in range(L - 1):
-        hb = hw * w[l] - (6 * w[l] - 12 if c16[l] and int(tw[l]) == 16 else 0)
+        hb = hw * w[l] - (6 * w[l] - 12 if c16[l] and int(tw[l]) in (8, 16) else 0)
         cyc += (args.numpre + args.numpost) * args.spi * N[l] * hb
-        if mixed and npre >= 1 and int(tw[l]) == 16:
+        if mixed and npre >= 1 and int(tw[l]) in (8, 16):
             cyc -= 0.5 * N[l] * ((hw - 8) + (hw - 9)) * w[l]     # zero-start half-sweeps
         elif mixed or l > 0:
             cyc += N[l] * w[l]                                     # zeroed before use
-        h16 = bool(c16[l]) and int(tw[l]) == 16
+        h16 = bool(c16[l]) and int(tw[l]) in (8, 16)
         cyc += (2 * w[l] + 8 if h16 else 6 * w[l]) * N[l] + N[l + 1] * w[l + 1]   # restriction
         cyc += 2 * N[l] * w[l] + N[l + 1] * w[l + 1]               # prolongation + correction
     nc = N[L - 1]
@@ -551,7 +551,7 @@ def main():
         word = 4 if args.mode == "mixed" else 8
         words = 9 if args.omega == 1.0 else 10
         tm = int(s.tile_widths()[1 if word == 4 else 0][0])
-        h16 = word == 4 and tm == 16 and bool(s.coef16_levels()[0])
+        h16 = word == 4 and tm in (8, 16) and bool(s.coef16_levels()[0])
         bytes_per_cell = words * word - (6 * word - 12 if h16 else 0)
         per_launch_bytes = (cells / max(nl, 1)) * bytes_per_cell
         avg_s = ms / max(nl, 1) / 1e3
@@ -559,7 +559,8 @@ def main():
         rl = "true" if args.omega != 1.0 else "false"
         fp = "float" if word == 4 else "double"
         fz = "true" if s.fused_halos()[1 if word == 4 else 0][0] else "false"
-        key = (f"k_zebra_seg<{fp}, 1, {rl}, 0, {'true' if h16 else 'false'}>" if tm == 16
+        segk = {16: "k_zebra_seg", 8: "k_zebra_seg8"}.get(tm)
+        key = (f"{segk}<{fp}, 1, {rl}, 0, {'true' if h16 else 'false'}>" if segk
                else f"k_zebra_line<{fp}, 1, {tm}, {rl}, {fz}>")
         tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",

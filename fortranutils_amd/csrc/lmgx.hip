@@ -55,9 +55,9 @@ struct LevelDev {
     // per-block launches (the fast path's block-loop mode): d_lb1[b] is block b alone, its tile
     // offsets rebased to 0; bt_*[b] its tile counts in each tiling
     LevBlk *d_lb1 = nullptr;
-    std::vector<long> bt_sg, bt_z64, bt_z128, bt_rr, bt_rp;
+    std::vector<long> bt_sg, bt_sg8, bt_z64, bt_z128, bt_rr, bt_rp;
     long tiles_int = 0, tiles_all = 0, tiles_rb = 0, tiles_ln = 0, nlines = 0;
-    long tiles_z64 = 0, tiles_z128 = 0, tiles_rr = 0, tiles_rp = 0, tiles_sg = 0;
+    long tiles_z64 = 0, tiles_z128 = 0, tiles_rr = 0, tiles_rp = 0, tiles_sg = 0, tiles_sg8 = 0;
     int njmax = 0;                      // longest j-line (interior rows) of the level
     int smax = 0;                       // largest hyperplane index i+j+k (1-based)
     std::vector<Face> h_fc;
@@ -199,7 +199,8 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
     for (int l = 0; l < L; ++l) {
         LevelDev &D = h->lev[l];
         D.h_lb.resize(h->nblk);
-        long ti = 0, ta = 0, tr = 0, tl = 0, tl2 = 0, tl4 = 0, trr = 0, trp = 0, tsg = 0, sl = 0, ns = 0;
+        long ti = 0, ta = 0, tr = 0, tl = 0, tl2 = 0, tl4 = 0, trr = 0, trp = 0, tsg = 0, tsg8 = 0;
+        long sl = 0, ns = 0;
         double np = 0.0;
         long npi = 0;
         int smax = 0;
@@ -246,6 +247,9 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             }
             B.t_sg = tsg;
             tsg += (long)((B.hp + 63) / 64) * ((n[2] - 2 + B.sg_s - 1) / B.sg_s);
+            B.sg_s8 = B.sg_n <= 8 ? 8 / B.sg_n : 0;
+            B.t_sg8 = tsg8;
+            if (B.sg_s8) tsg8 += (long)((B.hp + 63) / 64) * ((n[2] - 2 + B.sg_s8 - 1) / B.sg_s8);
             B.t_rp = trp;
             trp += (long)((((n[0] - 2) >> 1) + 63) / 64) * ((((n[2] - 2) >> 1) + 3) / 4) *
                    ((((n[1] - 2) >> 1) + LMGX_RCHP - 1) / LMGX_RCHP);
@@ -269,6 +273,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
         D.tiles_rr = trr;
         D.tiles_rp = trp;
         D.tiles_sg = tsg;
+        D.tiles_sg8 = tsg8;
         D.nlines = sl;
         D.smax = smax;
         D.npts_int = npi;
@@ -334,8 +339,10 @@ static int copy_tables(lmgx_handle *h, int l)
     LevelDev &D = h->lev[l];
     HIPCHK(hipMemcpy(D.d_lb, D.h_lb.data(), sizeof(LevBlk) * h->nblk, hipMemcpyHostToDevice));
     std::vector<LevBlk> one(D.h_lb);
-    for (LevBlk &B : one)
+    for (LevBlk &B : one) {
         B.t_int = B.t_all = B.t_rb = B.t_ln = B.t_z64 = B.t_z128 = B.t_rr = B.t_rp = B.t_sg = 0;
+        B.t_sg8 = 0;
+    }
     HIPCHK(hipMemcpy(D.d_lb1, one.data(), sizeof(LevBlk) * h->nblk, hipMemcpyHostToDevice));
     return 0;
 }
@@ -354,6 +361,7 @@ static int upload_tables(lmgx_handle *h)
                 out[b] = (b + 1 < h->nblk ? D.h_lb[b + 1].*f : total) - D.h_lb[b].*f;
         };
         span(&LevBlk::t_sg, D.tiles_sg, D.bt_sg);
+        span(&LevBlk::t_sg8, D.tiles_sg8, D.bt_sg8);
         span(&LevBlk::t_z64, D.tiles_z64, D.bt_z64);
         span(&LevBlk::t_z128, D.tiles_z128, D.bt_z128);
         span(&LevBlk::t_rr, D.tiles_rr, D.bt_rr);

@@ -38,6 +38,8 @@ struct LevBlk {
     int sg_n;        // segmented line solve: segments per line (power of two <= 16)
     int sg_r;        // rows per segment (<= LMGX_SMAX)
     int sg_s;        // line sets (k-planes) per workgroup = 16 / sg_n
+    long t_sg8;      // the same tiling for 8-wavefront workgroups (valid where sg_n <= 8)
+    int sg_s8;       // line sets per 8-wavefront workgroup = 8 / sg_n (0 when sg_n = 16)
     long s_jk;       // first (j,k) interior line of this block (hyperplane sweep)
     long o2;         // offset of the block's 2-D (i,j) metric arrays
     long o1;         // offset of the block's 1-D k metric arrays
@@ -99,7 +101,8 @@ __device__ inline int find_block(const LevBlk *lb, int nblk, long t, int which)
         long s = which == 0 ? lb[mid].t_int : which == 1 ? lb[mid].t_all : which == 2 ? lb[mid].t_rb
                  : which == 3 ? lb[mid].s_jk : which == 4 ? lb[mid].t_ln
                  : which == 5 ? lb[mid].t_z64 : which == 6 ? lb[mid].t_z128
-                 : which == 7 ? lb[mid].t_rr : which == 8 ? lb[mid].t_rp : lb[mid].t_sg;
+                 : which == 7 ? lb[mid].t_rr : which == 8 ? lb[mid].t_rp
+                 : which == 9 ? lb[mid].t_sg : lb[mid].t_sg8;
         if (s <= t) lo = mid; else hi = mid - 1;
     }
     return lo;

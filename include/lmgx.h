@@ -221,8 +221,10 @@ int lmgx_transpose_operator_device(lmgx_handle *h, double *const d_out[7]);
 int lmgx_rho_adjoint_device(lmgx_handle *h, const double *d_lambda, const double *d_p,
                             double *d_glogrho);
 
-/* Line-tile widths chosen by the fast path's autotuner: out[precision][LMGX_MAXL], precision 0 =
- * FP64, 1 = FP32; 0 for levels that do not exist or before the first fast solve. */
+/* Line kernels chosen by the fast path's autotuner: out[precision][LMGX_MAXL], precision 0 =
+ * FP64, 1 = FP32; 64 / 128 = two-ended line solve with that many lines per wavefront row, 16 =
+ * segmented solve in 16-wavefront workgroups, 8 = segmented solve in 8-wavefront workgroups; 0
+ * for levels that do not exist or before the first fast solve. */
 int lmgx_tile_widths(const lmgx_handle *h, int *out);
 
 /* Levels whose face-interior halos the fast path's line kernels store themselves instead of a

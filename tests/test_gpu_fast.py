@@ -241,7 +241,7 @@ def test_per_block_launches_agree(cfg, scale, prec, monkeypatch):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("prec,kernel", [(1, "seg"), (1, "128"), (0, "64")])
+@pytest.mark.parametrize("prec,kernel", [(1, "seg"), (1, "seg8"), (1, "128"), (0, "64")])
 @pytest.mark.parametrize("cfg,scale", [("c1", 2), ("c3", 4), ("c0", 1)])
 def test_fused_halo_stores_match_swaps(cfg, scale, prec, kernel, monkeypatch):
     """The line kernels (segmented, two-ended) storing the i- and k-face halos themselves give
@@ -266,10 +266,37 @@ def test_fused_halo_stores_match_swaps(cfg, scale, prec, kernel, monkeypatch):
         # every level it is valid for
         assert (fz[:-1] == (0 if mode == "1" else 1)).all()
         assert mode == "0" or not fz.any()
-        assert (tw[prec][:-1] == (16 if kernel == "seg" else int(kernel))).all()
+        want = {"seg": 16, "seg8": 8}.get(kernel) or int(kernel)
+        assert (tw[prec][:-1] == want).all()
         out.append((f.cpu().numpy(), hist, res.cpu().numpy()))
     for x, y in zip(out[0], out[1]):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c1", 2), ("c0", 1)])
+def test_eight_wavefront_segmented_solve_is_the_same_solve(cfg, scale, monkeypatch):
+    """The segmented line solve in 8-wavefront workgroups (lines of at most 8 segments) cuts the
+    lines into the same segments and solves them with the same arithmetic as in 16-wavefront
+    workgroups; only the distribution over workgroups differs: bitwise the same pressure,
+    residual and history, mixed precision, with and without the zero-start half-sweeps."""
+    import torch
+    out = []
+    for kernel in ("seg", "seg8"):
+        monkeypatch.setenv("LMGX_LINE_KERNEL", kernel)
+        p, o, s, coeffs, b = setup(cfg, scale)
+        for numpre in (1, 0):
+            f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+            res = torch.zeros_like(f)
+            hist = s.lmg_solve(f, b, res, max_cycles=4, tolf=0.0, rtol=0.0, smoother=1,
+                               precision=1, numpre=numpre, numpost=1, omega=1.35, omega_c=1.0,
+                               sweeps_per_iter=3)
+            out.append((kernel, numpre, f.cpu().numpy(), hist, res.cpu().numpy()))
+        tw = s.tile_widths()
+        print(cfg, kernel, "widths", tw.tolist())
+        assert (tw[1][:-1] == (16 if kernel == "seg" else 8)).all()
+    for a, c in zip(out[:2], out[2:]):
+        for x, y in zip(a[2:], c[2:]):
+            assert np.array_equal(x, y)
 
 
 @pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c1", 2), ("c2", 4)])
