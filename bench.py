@@ -467,7 +467,8 @@ def traffic(d):
         # every finest-level line-kernel instance (tile width, relaxation) separately
         names = sorted({n.replace("void ", "").split("(")[0] for n, _ in fetch_rows
                         if n.replace("void ", "").startswith((f"k_zebra_line<{word}, 1,",
-                                                              f"k_zebra_seg<{word}, 1,"))})
+                                                              f"k_zebra_seg<{word}, 1,",
+                                                              f"k_zebra_seg8<{word}, 1,"))})
         ent = {}
         for kname in names:
             pick = lambda name, k=kname: name.replace("void ", "").split("(")[0] == k
