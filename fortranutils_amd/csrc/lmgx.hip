@@ -72,6 +72,9 @@ struct FastBufs;  // split-layout storage of the fast path (kernels_fast.inc)
 
 struct lmgx_handle {
     int nblk = 0, nlev = 0;
+    // the operator came from the builder (lmgx_build_operator / lmgx_update_density_device), and
+    // whether the adjoint has swapped in its transpose for the moment
+    bool op_builder = false, op_flipped = false;
     int nijkp1[LMGX_MAXLEV][LMGX_MAXB][3];
     long off[LMGX_MAXLEV][LMGX_MAXB];
     double npts[LMGX_MAXLEV];
@@ -512,6 +515,7 @@ extern "C" int lmgx_set_operator(lmgx_handle *h, const double *ap, const double 
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_op = true;
     h->aT_valid = false;
+    h->op_builder = false;
     hipFree(h->d_rho_op);
     h->d_rho_op = nullptr;
     return fast_operator_changed(h);
@@ -527,6 +531,7 @@ extern "C" int lmgx_set_operator_device(lmgx_handle *h, const double *const d_co
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_op = true;
     h->aT_valid = false;
+    h->op_builder = false;
     hipFree(h->d_rho_op);
     h->d_rho_op = nullptr;
     return fast_operator_changed(h);

@@ -273,6 +273,34 @@ def test_fused_halo_stores_match_swaps(cfg, scale, prec, kernel, monkeypatch):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c4", 4), ("c0", 1)])
+def test_formed_diagonal_is_the_stored_one(cfg, scale, prec, monkeypatch):
+    """On the builder's operator the level-1 residual forms an interior cell's diagonal from its
+    couplings (the builder's own sum) instead of reading it; the same coefficients handed over
+    with lmgx_set_operator are read as they are.  Both solves give bitwise the same pressure,
+    residual and history (face-Dirichlet sides, interfaces and periodic sides included).  The
+    line kernel is pinned: the two solvers are tuned apart."""
+    import torch
+    from fortranutils_amd import lmgx
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg")
+    p, o, a, coeffs, b = setup(cfg, scale)
+    c = lmgx.Solver(lmgx.Topology.from_problem(p))
+    dirich = np.array([bl.dirichlet for bl in p.blocks], np.int32)
+    c.set_dirichlet(dirich)
+    c.set_operator(coeffs)
+    out = []
+    for s in (a, c):
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b.clone(), res, max_cycles=4, tolf=0.0, rtol=0.0, smoother=1,
+                           precision=prec, numpre=1, numpost=1, omega=1.35, omega_c=1.0,
+                           sweeps_per_iter=3)
+        out.append((f.cpu().numpy(), hist, res.cpu().numpy()))
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c1", 2), ("c0", 1)])
 def test_eight_wavefront_segmented_solve_is_the_same_solve(cfg, scale, monkeypatch):
     """The segmented line solve in 8-wavefront workgroups (lines of at most 8 segments) cuts the
