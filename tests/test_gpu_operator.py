@@ -39,12 +39,14 @@ def test_build_operator_bitwise(cfg, scale, ext):
 
 
 @pytest.mark.parametrize("cfg,scale", [("c4", 8), ("c3", 8)])
-def test_update_density_matches_build(cfg, scale):
+def test_update_density_matches_build(cfg, scale, monkeypatch):
     """lmgx_update_density_device (coefficients rebuilt on the device for a new density) gives
     bitwise the operator of lmgx_build_operator with that density, and of the oracle; a fast
-    solve after the update equals a fast solve on a freshly built operator bitwise."""
+    solve after the update equals a fast solve on a freshly built operator bitwise (the line
+    kernel pinned: each solver tunes its own, and kernels that differ are different relaxations)."""
     import torch
     from fortranutils_amd import lmgx
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg")
     p = P.config(cfg, scale)
     o = O.Oracle(p)
     geo = o.geometry()
