@@ -114,6 +114,13 @@ struct lmgx_handle {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
     std::vector<double> prof_ev_cells;
     size_t prof_used = 0;
+    // work counters (lmgx_counters): operator uploads, fast-path operator preparations, cycle
+    // graph captures, solves
+    long n_uploads = 0, n_prepares = 0, n_captures = 0, n_solves = 0;
+    // lmgx_set_operator_cached: the host arrays of its last upload and their sampled fingerprint
+    const double *cached_src[7] = {nullptr};
+    unsigned long long cached_fp = 0;
+    bool cached_valid = false;
 };
 
 static const int CYC[2][3] = {{2, 3, 1}, {3, 1, 2}};
@@ -516,9 +523,56 @@ extern "C" int lmgx_set_operator(lmgx_handle *h, const double *ap, const double 
     h->have_op = true;
     h->aT_valid = false;
     h->op_builder = false;
+    h->cached_valid = false;
+    ++h->n_uploads;
     hipFree(h->d_rho_op);
     h->d_rho_op = nullptr;
     return fast_operator_changed(h);
+}
+
+// 64-bit FNV-1a over the bits of every 4093rd element (and the last) of the seven arrays
+static unsigned long long operator_fingerprint(const double *const src[7], long n)
+{
+    unsigned long long fp = 1469598103934665603ull;
+    for (int q = 0; q < 7; ++q)
+        for (long i = 0; i < n; i = (i + 4093 < n || i == n - 1) ? i + 4093 : n - 1) {
+            unsigned long long v;
+            memcpy(&v, src[q] + i, sizeof(v));
+            fp = (fp ^ v) * 1099511628211ull;
+        }
+    return fp;
+}
+
+extern "C" int lmgx_set_operator_cached(lmgx_handle *h, const double *ap, const double *aw,
+                                        const double *ae, const double *as, const double *an,
+                                        const double *al, const double *ar, int force,
+                                        int *uploaded)
+{
+    if (!h) return fail(LMGX_E_ARG, "NULL handle");
+    const double *src[7] = {ap, aw, ae, as, an, al, ar};
+    for (int q = 0; q < 7; ++q)
+        if (!src[q]) return fail(LMGX_E_ARG, "NULL coefficient array");
+    if (uploaded) *uploaded = 0;
+    const unsigned long long fp = operator_fingerprint(src, h->ntot);
+    bool same = h->cached_valid && h->have_op && !h->op_flipped && fp == h->cached_fp && !force;
+    for (int q = 0; q < 7; ++q) same = same && src[q] == h->cached_src[q];
+    if (same) return 0;
+    CHK(lmgx_set_operator(h, ap, aw, ae, as, an, al, ar));
+    for (int q = 0; q < 7; ++q) h->cached_src[q] = src[q];
+    h->cached_fp = fp;
+    h->cached_valid = true;
+    if (uploaded) *uploaded = 1;
+    return 0;
+}
+
+extern "C" int lmgx_counters(const lmgx_handle *h, long out[4])
+{
+    if (!h || !out) return fail(LMGX_E_ARG, "NULL argument");
+    out[0] = h->n_uploads;
+    out[1] = h->n_prepares;
+    out[2] = h->n_captures;
+    out[3] = h->n_solves;
+    return 0;
 }
 
 extern "C" int lmgx_set_operator_device(lmgx_handle *h, const double *const d_coeff[7])
@@ -532,6 +586,8 @@ extern "C" int lmgx_set_operator_device(lmgx_handle *h, const double *const d_co
     h->have_op = true;
     h->aT_valid = false;
     h->op_builder = false;
+    h->cached_valid = false;
+    ++h->n_uploads;
     hipFree(h->d_rho_op);
     h->d_rho_op = nullptr;
     return fast_operator_changed(h);
@@ -615,21 +671,39 @@ static int swap_level(lmgx_handle *h, int l, T *x)
 static void prof_begin(lmgx_handle *h);
 static void prof_end(lmgx_handle *h, double cells);
 
+static void sor_sweep(lmgx_handle *h, int l, double *f, const double *b, double omega, int d1)
+{
+    LevelDev &D = h->lev[l];
+    double *const *a = h->d_a;
+    prof_begin(h);
+    for (int s = 6; s <= D.smax; ++s)
+        hipLaunchKernelGGL(k_sor_hyperplane, dim3(nblocks(D.nlines)), dim3(LMGX_WG), 0, h->stream,
+                           D.d_lb, h->nblk, D.nlines, s, d1, omega, a[0], a[1], a[2], a[3], a[4],
+                           a[5], a[6], f, b);
+    prof_end(h, (double)D.npts_int);
+}
+
+// one sporlx2 call (meth_smooth = 2): 7 sweeps in natural order, i.e. spor with
+// (io, jo, ko) = (1, nip1, nip1*njp1) and the d1 = 1 summation order, a halo swap after each;
+// the SAVEd jumble is not used (code/sporlx2.F:64-146)
+static int sporlx2_exact(lmgx_handle *h, int l, double *f, const double *b, double omega)
+{
+    for (int il = 0; il < 7; ++il) {
+        sor_sweep(h, l, f, b, omega, 1);
+        CHK(swap_level<double>(h, l, f));
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 // one sporlx call: 3 sweeps, halo swap after each (code/sporlx.F:74-159)
 static int sporlx_exact(lmgx_handle *h, int l, double *f, const double *b, double omega)
 {
     int jumfor = (h->jumble % 16 >= 8) ? 1 : 2;
     int jumst = (h->jumble % 24) / 8 + 1;
     int d[3] = {jumst, CYC[(jumfor + 1) % 2][jumst - 1], CYC[jumfor % 2][jumst - 1]};
-    LevelDev &D = h->lev[l];
-    double *const *a = h->d_a;
     for (int il = 0; il < 3; ++il) {
-        prof_begin(h);
-        for (int s = 6; s <= D.smax; ++s)
-            hipLaunchKernelGGL(k_sor_hyperplane, dim3(nblocks(D.nlines)), dim3(LMGX_WG), 0,
-                               h->stream, D.d_lb, h->nblk, D.nlines, s, d[il], omega, a[0], a[1],
-                               a[2], a[3], a[4], a[5], a[6], f, b);
-        prof_end(h, (double)D.npts_int);
+        sor_sweep(h, l, f, b, omega, d[il]);
         CHK(swap_level<double>(h, l, f));
     }
     HIPCHK(hipGetLastError());
@@ -659,11 +733,15 @@ static int prolong_exact(lmgx_handle *h, int lf, double *f)
 }
 
 // code/smooth.F:34-100 (the smoothing stop test never fires with soltol = 0 unless a sweep
-// leaves f exactly unchanged; this path always runs the full count)
+// leaves f exactly unchanged; this path always runs the full count).  smoother selects
+// meth_smooth: LMGX_SMOOTH_EXACT = sporlx (1), LMGX_SMOOTH_SPORLX2 = sporlx2 (2)
 static int smooth_exact(lmgx_handle *h, int l, double *f, double *b, double *res, int nsmooth,
-                        double omega, double *d_nrm)
+                        double omega, double *d_nrm, int smoother)
 {
-    for (int it = 0; it < nsmooth; ++it) CHK(sporlx_exact(h, l, f, b, omega));
+    for (int it = 0; it < nsmooth; ++it) {
+        if (smoother == LMGX_SMOOTH_SPORLX2) CHK(sporlx2_exact(h, l, f, b, omega));
+        else CHK(sporlx_exact(h, l, f, b, omega));
+    }
     return norms_exact(h, l, f, b, res, d_nrm);
 }
 
@@ -705,20 +783,23 @@ static int solve_exact(lmgx_handle *h, double *f, double *b, double *res, const 
             if (goto_fine) {
                 if (il == L - 1) {
                     CHK(norms_exact(h, il, f, b, res, d_nrm + 4 * il));
-                    CHK(smooth_exact(h, il, f, b, res, p->numcits, p->omega_c, d_nrm + 4 * il));
+                    CHK(smooth_exact(h, il, f, b, res, p->numcits, p->omega_c, d_nrm + 4 * il,
+                                     p->smoother));
                     if (fcycle) gamma = 1;
                 }
                 if (il > 0 && L != 1) {
                     il = il - 1;
                     CHK(prolong_exact(h, il, f));
                     if (il == 0) CHK(norms_exact(h, il, f, b, res, d_nrm));
-                    CHK(smooth_exact(h, il, f, b, res, p->numpost, p->omega, d_nrm + 4 * il));
+                    CHK(smooth_exact(h, il, f, b, res, p->numpost, p->omega, d_nrm + 4 * il,
+                                     p->smoother));
                     if (il == 0 && fcycle) gamma = 2;
                 }
                 nummg[il] = nummg[il] - 1;
             } else {
                 if (il == 0) CHK(norms_exact(h, il, f, b, res, d_nrm));
-                CHK(smooth_exact(h, il, f, b, res, p->numpre, p->omega, d_nrm + 4 * il));
+                CHK(smooth_exact(h, il, f, b, res, p->numpre, p->omega, d_nrm + 4 * il,
+                                 p->smoother));
                 il = il + 1;
                 nummg[il] = gamma;
                 CHK(restrict_exact(h, il, res, b, f));
@@ -752,8 +833,10 @@ static int check_params(const lmgx_handle *h, const lmgx_params *p)
     if (p->cycle != 1 && p->cycle != 2) return fail(LMGX_E_ARG, "cycle must be 1 (V) or 2 (F)");
     if (p->numpre < 0 || p->numpost < 0 || p->numcits < 0 || p->max_cycles < 0)
         return fail(LMGX_E_ARG, "negative iteration count");
-    if (p->smoother != LMGX_SMOOTH_EXACT && p->smoother != LMGX_SMOOTH_LINE)
-        return fail(LMGX_E_ARG, "unknown smoother");
+    if (p->smoother != LMGX_SMOOTH_EXACT && p->smoother != LMGX_SMOOTH_LINE &&
+        p->smoother != LMGX_SMOOTH_SPORLX2)
+        return fail(LMGX_E_UNSUPPORTED, "smoothing method " + std::to_string(p->smoother) +
+                                            " not implemented (code/smooth.F:77-79)");
     if (p->precision != LMGX_PREC_FP64 && p->precision != LMGX_PREC_MIXED)
         return fail(LMGX_E_ARG, "unknown precision");
     if (p->precision == LMGX_PREC_MIXED && p->smoother != LMGX_SMOOTH_LINE)
@@ -771,8 +854,9 @@ extern "C" int lmgx_solve_device(lmgx_handle *h, double *d_f, double *d_b, doubl
     CHK(check_params(h, prm));
     int nc = 0;
     CHK(ensure_hist(h, (long)std::max(1, prm->max_cycles) * h->nlev * 4 + 64));
-    int rc = prm->smoother == LMGX_SMOOTH_EXACT ? solve_exact(h, d_f, d_b, d_res, prm, history, &nc)
-                                                : solve_fast(h, d_f, d_b, d_res, prm, history, &nc);
+    ++h->n_solves;
+    int rc = prm->smoother != LMGX_SMOOTH_LINE ? solve_exact(h, d_f, d_b, d_res, prm, history, &nc)
+                                               : solve_fast(h, d_f, d_b, d_res, prm, history, &nc);
     if (ncycles_out) *ncycles_out = nc;
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -790,14 +874,19 @@ extern "C" int lmgx_solve(lmgx_handle *h, double *f, double *b, double *residu,
         HIPCHK(hipMalloc(&h->d_b, bytes));
         HIPCHK(hipMalloc(&h->d_res, bytes));
     }
-    HIPCHK(hipMemcpyAsync(h->d_f, f, bytes, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->d_b, b, bytes, hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->d_res, residu, bytes, hipMemcpyHostToDevice, h->stream));
+    // The reference-order path writes f, b (levels >= 2) and residu on every level.  The line
+    // path reads f and b and writes f and residu on level 1 only (its coarse levels live in the
+    // split layout), so only level 1 travels, and b does not come back.
+    const bool line = prm->smoother == LMGX_SMOOTH_LINE;
+    const size_t part = line ? sizeof(double) * h->n1 : bytes;
+    HIPCHK(hipMemcpyAsync(h->d_f, f, part, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_b, b, part, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_res, residu, part, hipMemcpyHostToDevice, h->stream));
     int rc = lmgx_solve_device(h, h->d_f, h->d_b, h->d_res, prm, history, ncycles_out);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(f, h->d_f, bytes, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(b, h->d_b, bytes, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(residu, h->d_res, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(f, h->d_f, part, hipMemcpyDeviceToHost, h->stream));
+    if (!line) HIPCHK(hipMemcpyAsync(b, h->d_b, part, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(residu, h->d_res, part, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -831,6 +920,16 @@ extern "C" int lmgx_sporlx_device(lmgx_handle *h, int level, double *d_f, const 
     CHK(check_level(h, level));
     if (!h->have_op) return fail(LMGX_E_STATE, "no operator");
     CHK(sporlx_exact(h, level, d_f, d_b, omega));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int lmgx_sporlx2_device(lmgx_handle *h, int level, double *d_f, const double *d_b,
+                                   double omega)
+{
+    CHK(check_level(h, level));
+    if (!h->have_op) return fail(LMGX_E_STATE, "no operator");
+    CHK(sporlx2_exact(h, level, d_f, d_b, omega));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -903,6 +1002,29 @@ extern "C" int lmgx_global_sum_device(lmgx_handle *h, int level, const double *d
                           hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     memcpy(out, h->h_pinned, 3 * sizeof(double));
+    return 0;
+}
+
+extern "C" int lmgx_p_mean_subtract_device(lmgx_handle *h, double *d_p, double *mean_out)
+{
+    if (!h || !d_p) return fail(LMGX_E_ARG, "NULL argument");
+    LevelDev &D = h->lev[0];
+    // the level-1 interior sum in the fixed two-stage order of k_gsum, then mean = sum / npts
+    // and p -= mean over the whole level, halos included (as p_ref_subtract shifts them)
+    hipLaunchKernelGGL(k_gsum, dim3(D.tiles_int), dim3(LMGX_WG), 0, h->stream, D.d_lb, h->nblk,
+                       (const double *)d_p, h->d_partial);
+    hipLaunchKernelGGL(k_gsum_final, dim3(1), dim3(LMGX_WG), 0, h->stream,
+                       (const double *)h->d_partial, D.tiles_int, h->d_scalar + 44);
+    hipLaunchKernelGGL(k_mean_of, dim3(1), dim3(64), 0, h->stream,
+                       (const double *)(h->d_scalar + 44), h->npts[0], h->d_scalar + 47);
+    hipLaunchKernelGGL(k_sub_scalar, dim3(nblocks(h->n1)), dim3(LMGX_WG), 0, h->stream, d_p, h->n1,
+                       (const double *)(h->d_scalar + 47));
+    HIPCHK(hipGetLastError());
+    CHK(ensure_hist(h, 64));
+    HIPCHK(hipMemcpyAsync(h->h_pinned, h->d_scalar + 47, sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (mean_out) memcpy(mean_out, h->h_pinned, sizeof(double));
     return 0;
 }
 

@@ -22,7 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblmgx.so")
 
-LMGX_SMOOTH_EXACT, LMGX_SMOOTH_LINE = 0, 1
+LMGX_SMOOTH_EXACT, LMGX_SMOOTH_LINE, LMGX_SMOOTH_SPORLX2 = 0, 1, 2
 LMGX_PREC_FP64, LMGX_PREC_MIXED = 0, 1
 MAXL = 8
 
@@ -73,21 +73,42 @@ def _check(rc: int):
         raise LmgxError(f"lmgx error {rc}: {lib().lmgx_last_error().decode()}")
 
 
-def _dp(a) -> C.POINTER(C.c_double):
-    """Pointer to a float64 numpy array or a torch tensor (host or device).
+def _hp(a, n: int, what: str = "array") -> C.POINTER(C.c_double):
+    """Pointer to a host float64 numpy array of at least ``n`` elements (C order); anything else
+    raises :class:`LmgxError` before the library is called."""
+    if not isinstance(a, np.ndarray):
+        raise LmgxError(f"{what}: a numpy float64 array is required, got {type(a).__name__}")
+    if a.dtype != np.float64:
+        raise LmgxError(f"{what}: dtype {a.dtype}, float64 is required")
+    if not a.flags.c_contiguous:
+        raise LmgxError(f"{what}: the array must be C-contiguous")
+    if a.size < n:
+        raise LmgxError(f"{what}: {a.size} elements, at least {n} are required")
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _dp(a, n: int, what: str = "tensor") -> C.POINTER(C.c_double):
+    """Pointer to a contiguous float64 CUDA tensor of at least ``n`` elements; a CPU tensor, a
+    numpy array, another dtype or a short tensor raises :class:`LmgxError` before the library is
+    called (the library itself cannot check device pointers).
 
     The library works on its own HIP stream, so a device tensor is handed over only after the
     work torch has queued on its current stream (which may still be writing the tensor) is done.
     The library synchronises its own stream before it returns, so its results are complete for
     torch."""
-    if isinstance(a, np.ndarray):
-        assert a.dtype == np.float64 and a.flags.c_contiguous
-        return a.ctypes.data_as(C.POINTER(C.c_double))
-    # torch tensor
-    assert a.is_contiguous(), "tensor must be contiguous"
-    if a.is_cuda:
-        import torch
-        torch.cuda.current_stream(a.device).synchronize()
+    import torch
+    if not isinstance(a, torch.Tensor):
+        raise LmgxError(f"{what}: a float64 CUDA tensor is required, got {type(a).__name__}")
+    if not a.is_cuda:
+        raise LmgxError(f"{what}: the tensor is on {a.device}; device entry points need a CUDA "
+                        "tensor (host arrays go as numpy)")
+    if a.dtype != torch.float64:
+        raise LmgxError(f"{what}: dtype {a.dtype}, torch.float64 is required")
+    if not a.is_contiguous():
+        raise LmgxError(f"{what}: the tensor must be contiguous")
+    if a.numel() < n:
+        raise LmgxError(f"{what}: {a.numel()} elements, at least {n} are required")
+    torch.cuda.current_stream(a.device).synchronize()
     return C.cast(C.c_void_p(a.data_ptr()), C.POINTER(C.c_double))
 
 
@@ -142,6 +163,9 @@ class Solver:
         self.nlev, self.ntot, self.n1, self.nblk = nlev.value, ntot.value, n1.value, nb
         self.nijkp1 = dims[: self.nlev].copy()
         self.off = off[: self.nlev].copy()
+        # multi-level 2-D (ij) and 1-D (k) metric arrays, reference code/set_offset.F:166-234
+        self.n2d = int(np.sum(self.nijkp1[..., 0].astype(np.int64) * self.nijkp1[..., 1]))
+        self.n1d = int(np.sum(self.nijkp1[..., 2]))
 
     def close(self):
         if getattr(self, "h", None):
@@ -157,11 +181,16 @@ class Solver:
     # ---- operator -----------------------------------------------------------------------
     def set_operator(self, coeffs: Sequence):
         """ap, aw, ae, as, an, al, ar (ntot each): numpy (host) or torch CUDA tensors."""
+        if len(coeffs) != 7:
+            raise LmgxError(f"set_operator: 7 coefficient arrays are required, got {len(coeffs)}")
+        names = ("ap", "aw", "ae", "as", "an", "al", "ar")
         if isinstance(coeffs[0], np.ndarray):
-            _check(lib().lmgx_set_operator(self.h, *[_dp(np.ascontiguousarray(a, np.float64))
-                                                     for a in coeffs]))
+            host = [np.ascontiguousarray(a, np.float64) for a in coeffs]
+            _check(lib().lmgx_set_operator(self.h, *[_hp(a, self.ntot, q)
+                                                     for a, q in zip(host, names)]))
         else:
-            arr = (C.POINTER(C.c_double) * 7)(*[_dp(a) for a in coeffs])
+            arr = (C.POINTER(C.c_double) * 7)(*[_dp(a, self.ntot, q)
+                                                for a, q in zip(coeffs, names)])
             _check(lib().lmgx_set_operator_device(self.h, arr))
 
     def build_operator(self, problem, rho1: Optional[np.ndarray] = None, dirichlet=None):
@@ -173,14 +202,16 @@ class Solver:
         zv = np.concatenate([b.zv for b in problem.blocks]).astype(np.float64)
         r = None if rho1 is None else np.ascontiguousarray(rho1, np.float64)
         d = None if dirichlet is None else np.ascontiguousarray(dirichlet, np.int32).ravel()
-        _check(lib().lmgx_build_operator(self.h, _dp(xv), _dp(yv), _dp(zv),
-                                         None if r is None else _dp(r),
+        _check(lib().lmgx_build_operator(self.h, _hp(xv, xv.size), _hp(yv, yv.size),
+                                         _hp(zv, zv.size),
+                                         None if r is None else _hp(r, self.n1, "rho1"),
                                          None if d is None else _ip(d)))
 
     def update_density(self, rho1=None):
         """Rebuild the coefficients of every level on the device for a new level-1 density (a
         device tensor of n1 with halos; None = 1), keeping the metrics and Dirichlet sides."""
-        _check(lib().lmgx_update_density_device(self.h, None if rho1 is None else _dp(rho1)))
+        _check(lib().lmgx_update_density_device(
+            self.h, None if rho1 is None else _dp(rho1, self.n1, "rho1")))
 
     def set_dirichlet(self, flags):
         f = None if flags is None else np.ascontiguousarray(flags, np.int32).ravel()
@@ -188,12 +219,14 @@ class Solver:
 
     def get_operator(self):
         a = [np.zeros(self.ntot) for _ in range(7)]
-        _check(lib().lmgx_get_operator(self.h, *[_dp(x) for x in a]))
+        _check(lib().lmgx_get_operator(self.h, *[_hp(x, self.ntot) for x in a]))
         return a
 
     def source(self, u, v, w, rho, rhonp1, dtime, b):
-        _check(lib().lmgx_source_device(self.h, _dp(u), _dp(v), _dp(w), _dp(rho), _dp(rhonp1),
-                                        C.c_double(dtime), _dp(b)))
+        n = self.n1
+        _check(lib().lmgx_source_device(self.h, _dp(u, n, "u"), _dp(v, n, "v"), _dp(w, n, "w"),
+                                        _dp(rho, n, "rho"), _dp(rhonp1, n, "rhonp1"),
+                                        C.c_double(dtime), _dp(b, n, "b")))
 
     def uvw_adv(self, ivar, u, u_int, rho, p, dtime, diu=None, rp=None, dkw=None):
         """Velocity correction after the solve (reference ``code/uvw_adv.F:5``): component ``ivar``
@@ -204,12 +237,19 @@ class Solver:
         if isinstance(u, np.ndarray):
             if diu is None or rp is None or dkw is None:
                 raise LmgxError("host uvw_adv needs the metrics diu (djv for V), rp and dkw")
-            _check(lib().lmgx_uvw_adv(self.h, int(ivar), _dp(diu), _dp(rp), _dp(dkw), _dp(u),
-                                      _dp(u_int), _dp(rho), _dp(p), C.c_double(dtime)))
+            n = self.n1
+            _check(lib().lmgx_uvw_adv(self.h, int(ivar), _hp(diu, self.n2d, "diu"),
+                                      _hp(rp, self.n2d, "rp"), _hp(dkw, self.n1d, "dkw"),
+                                      _hp(u, n, "u"), _hp(u_int, n, "u_int"), _hp(rho, n, "rho"),
+                                      _hp(p, n, "p"), C.c_double(dtime)))
         else:
-            m = [None if a is None else _dp(a) for a in (diu, rp, dkw)]
-            _check(lib().lmgx_uvw_adv_device(self.h, int(ivar), *m, _dp(u), _dp(u_int), _dp(rho),
-                                             _dp(p), C.c_double(dtime)))
+            m = [None if a is None else _dp(a, sz, q)
+                 for a, sz, q in ((diu, self.n2d, "diu"), (rp, self.n2d, "rp"),
+                                  (dkw, self.n1d, "dkw"))]
+            n = self.n1
+            _check(lib().lmgx_uvw_adv_device(self.h, int(ivar), *m, _dp(u, n, "u"),
+                                             _dp(u_int, n, "u_int"), _dp(rho, n, "rho"),
+                                             _dp(p, n, "p"), C.c_double(dtime)))
         return u
 
     # ---- field dumps (reference code/file_3.F, code/read_3.F) ------------------------------
@@ -217,14 +257,17 @@ class Solver:
         """Write level ``level`` of the device field ``f`` (multi-level natural layout) in the
         reference's 3-D file format: ``filefrmt`` 0 unformatted (REAL*4 values, or REAL*8 with
         ``real_kind=8``), 1 text."""
-        _check(lib().lmgx_file_3_device(self.h, os.fsencode(filename), int(filefrmt), _dp(f),
+        need = self.n1 if (int(level) == 0 or int(mg_or_sg) == 2) else self.ntot
+        _check(lib().lmgx_file_3_device(self.h, os.fsencode(filename), int(filefrmt),
+                                        _dp(f, need, "f"),
                                         int(level), int(mg_or_sg), int(real_kind)))
 
     def read_3(self, filename, f, filefrmt=0, real_kind=4) -> bool:
         """Read such a file into level 1 of the device field ``f``; a missing file gives zeros
         and returns False, as the reference does."""
         found = C.c_int()
-        _check(lib().lmgx_read_3_device(self.h, os.fsencode(filename), int(filefrmt), _dp(f),
+        _check(lib().lmgx_read_3_device(self.h, os.fsencode(filename), int(filefrmt),
+                                        _dp(f, self.n1, "f"),
                                         int(real_kind), C.byref(found)))
         return bool(found.value)
 
@@ -236,12 +279,15 @@ class Solver:
         p = default_params(max_cycles=max_cycles, **kw)
         hist = np.zeros((max(1, max_cycles), self.nlev, 4))
         nc = C.c_int()
+        n = self.ntot
+        hp = _hp(hist, hist.size)
         if isinstance(f, np.ndarray):
-            _check(lib().lmgx_solve(self.h, _dp(f), _dp(b), _dp(residu), C.byref(p), _dp(hist),
-                                    C.byref(nc)))
+            _check(lib().lmgx_solve(self.h, _hp(f, n, "f"), _hp(b, n, "b"),
+                                    _hp(residu, n, "residu"), C.byref(p), hp, C.byref(nc)))
         else:
-            _check(lib().lmgx_solve_device(self.h, _dp(f), _dp(b), _dp(residu), C.byref(p),
-                                           _dp(hist), C.byref(nc)))
+            _check(lib().lmgx_solve_device(self.h, _dp(f, n, "f"), _dp(b, n, "b"),
+                                           _dp(residu, n, "residu"), C.byref(p), hp,
+                                           C.byref(nc)))
         return hist[: nc.value]
 
     def solve_adjoint(self, lam, g, res, *, max_cycles=50, history=True, **kw):
@@ -250,15 +296,17 @@ class Solver:
         p = default_params(max_cycles=max_cycles, **kw)
         hist = np.zeros((max(1, max_cycles), self.nlev, 4))
         nc = C.c_int()
-        _check(lib().lmgx_solve_adjoint_device(self.h, _dp(lam), _dp(g), _dp(res), C.byref(p),
-                                               _dp(hist), C.byref(nc)))
+        n = self.ntot
+        _check(lib().lmgx_solve_adjoint_device(self.h, _dp(lam, n, "lam"), _dp(g, n, "g"),
+                                               _dp(res, n, "res"), C.byref(p),
+                                               _hp(hist, hist.size), C.byref(nc)))
         return hist[: nc.value]
 
     def transpose_operator(self):
         """The transposed operator, 7 device tensors of ntot (ap aw ae as an al ar)."""
         import torch
         out = [torch.zeros(self.ntot, dtype=torch.float64, device="cuda") for _ in range(7)]
-        arr = (C.POINTER(C.c_double) * 7)(*[_dp(a) for a in out])
+        arr = (C.POINTER(C.c_double) * 7)(*[_dp(a, self.ntot) for a in out])
         _check(lib().lmgx_transpose_operator_device(self.h, arr))
         return out
 
@@ -266,7 +314,8 @@ class Solver:
         """dJ/dlog(rho) (device tensor of n1) from level-1 p (A p = b) and lam (A^T lam = dJ/dp)."""
         import torch
         g = torch.zeros(self.n1, dtype=torch.float64, device="cuda")
-        _check(lib().lmgx_rho_adjoint_device(self.h, _dp(lam), _dp(p), _dp(g)))
+        _check(lib().lmgx_rho_adjoint_device(self.h, _dp(lam, self.n1, "lam"),
+                                             _dp(p, self.n1, "p"), _dp(g, self.n1)))
         return g
 
     @property
@@ -280,30 +329,46 @@ class Solver:
     # ---- single passes (device tensors) -------------------------------------------------
     def norms(self, level, x, b, res):
         out = np.zeros(4)
-        _check(lib().lmgx_norms_device(self.h, level, _dp(x), _dp(b), _dp(res), _dp(out)))
+        n = self.ntot
+        _check(lib().lmgx_norms_device(self.h, level, _dp(x, n, "x"), _dp(b, n, "b"),
+                                       _dp(res, n, "res"), _hp(out, 4)))
         return out
 
     def sporlx(self, level, f, b, omega):
-        _check(lib().lmgx_sporlx_device(self.h, level, _dp(f), _dp(b), C.c_double(omega)))
+        _check(lib().lmgx_sporlx_device(self.h, level, _dp(f, self.ntot, "f"),
+                                        _dp(b, self.ntot, "b"), C.c_double(omega)))
+
+    def sporlx2(self, level, f, b, omega):
+        _check(lib().lmgx_sporlx2_device(self.h, level, _dp(f, self.ntot, "f"),
+                                         _dp(b, self.ntot, "b"), C.c_double(omega)))
 
     def swap(self, level, var):
-        _check(lib().lmgx_swap_device(self.h, level, _dp(var)))
+        _check(lib().lmgx_swap_device(self.h, level, _dp(var, self.ntot, "var")))
 
     def restrict(self, coarse_level, res, b, f):
-        _check(lib().lmgx_restrict_device(self.h, coarse_level, _dp(res), _dp(b), _dp(f)))
+        n = self.ntot
+        _check(lib().lmgx_restrict_device(self.h, coarse_level, _dp(res, n, "res"),
+                                          _dp(b, n, "b"), _dp(f, n, "f")))
 
     def prolong(self, fine_level, f):
-        _check(lib().lmgx_prolong_device(self.h, fine_level, _dp(f)))
+        _check(lib().lmgx_prolong_device(self.h, fine_level, _dp(f, self.ntot, "f")))
 
     def extrap_p(self, p):
-        _check(lib().lmgx_extrap_p_device(self.h, _dp(p)))
+        _check(lib().lmgx_extrap_p_device(self.h, _dp(p, self.n1, "p")))
 
     def p_ref_subtract(self, p):
-        _check(lib().lmgx_p_ref_subtract_device(self.h, _dp(p)))
+        _check(lib().lmgx_p_ref_subtract_device(self.h, _dp(p, self.n1, "p")))
+
+    def p_mean_subtract(self, p) -> float:
+        """Mean-pressure gauge: subtract the level-1 interior mean from every level-1 point of
+        the device tensor ``p`` (halos included); returns the mean."""
+        m = C.c_double()
+        _check(lib().lmgx_p_mean_subtract_device(self.h, _dp(p, self.n1, "p"), C.byref(m)))
+        return m.value
 
     def global_sum(self, level, f):
         out = np.zeros(3)
-        _check(lib().lmgx_global_sum_device(self.h, level, _dp(f), _dp(out)))
+        _check(lib().lmgx_global_sum_device(self.h, level, _dp(f, self.ntot, "f"), _hp(out, 3)))
         return out
 
     def tile_widths(self):

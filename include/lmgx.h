@@ -51,6 +51,8 @@ typedef struct lmgx_topology {
 /* smoother */
 #define LMGX_SMOOTH_EXACT 0   /* reference order: 3 lexicographic SOR sweeps per iteration */
 #define LMGX_SMOOTH_LINE 1      /* zebra y-line relaxation (lines coloured by i+k), colour-split layout */
+#define LMGX_SMOOTH_SPORLX2 2   /* reference order, meth_smooth = 2: 7 natural-order SOR sweeps per
+                                   iteration, no jumble (code/sporlx2.F:11-161) */
 /* precision */
 #define LMGX_PREC_FP64 0
 #define LMGX_PREC_MIXED 1     /* FP32 V-cycle inside FP64 iterative refinement (needs SMOOTH_LINE) */
@@ -64,7 +66,9 @@ typedef struct lmgx_params {
     double omega_c;       /* relax_pmgc */
     double tolf;          /* reference stop test enorm/vnorm <= tolf (lmg_solve.F:82) */
     double rtol;          /* extra stop test rnorm_k/rnorm_0 <= rtol, 0 = off */
-    int smoother;         /* LMGX_SMOOTH_* */
+    int smoother;         /* LMGX_SMOOTH_*: EXACT and SPORLX2 are the reference's meth_smooth 1
+                             and 2 (code/smooth.F:62-73); another value returns
+                             LMGX_E_UNSUPPORTED where the reference stops (smooth.F:77-79) */
     int precision;        /* LMGX_PREC_* (MIXED needs LMGX_SMOOTH_LINE) */
     int check_every;      /* fast path: test convergence every N cycles (>= 1) */
     int sweeps_per_iter;  /* fast path: zebra line sweeps per smoothing iteration (reference SOR: 3) */
@@ -133,6 +137,13 @@ int lmgx_p_ref_subtract_device(lmgx_handle *h, double *d_p);
 /* code/global_sum.F:105-147: sum, max and min of f over the interior of level l. */
 int lmgx_global_sum_device(lmgx_handle *h, int level, const double *d_f, double out[3]);
 
+/* Mean-pressure gauge, the alternative to code/p_ref_subtract.F:6-100 (which subtracts the
+ * value at one cell): subtract the interior mean of level 1 (the FP64 sum of lmgx_global_sum
+ * divided by the interior count npts_all(1), code/set_loop_bound.F:101-102) from every point of
+ * level 1, halos included.  The reference has no such mode.  mean_out (may be NULL): the mean
+ * subtracted.  Device array of n1 (or ntot) doubles. */
+int lmgx_p_mean_subtract_device(lmgx_handle *h, double *d_p, double *mean_out);
+
 /* Single passes (device arrays, level 0-based), exposed for testing and for callers that drive
  * their own schedule: code/norms.F:5-126, code/sporlx.F:5-203 (one call = 3 sweeps),
  * code/swap_sc0.F:6-293, code/restriction.F:5-68 (+ init_array of the coarse f),
@@ -140,6 +151,8 @@ int lmgx_global_sum_device(lmgx_handle *h, int level, const double *d_f, double 
 int lmgx_norms_device(lmgx_handle *h, int level, const double *d_x, const double *d_b,
                       double *d_res, double norms_out[4]);
 int lmgx_sporlx_device(lmgx_handle *h, int level, double *d_f, const double *d_b, double omega);
+/* code/sporlx2.F:11-161 (one call = 7 natural-order sweeps, jumble untouched) */
+int lmgx_sporlx2_device(lmgx_handle *h, int level, double *d_f, const double *d_b, double omega);
 int lmgx_swap_device(lmgx_handle *h, int level, double *d_var);
 int lmgx_restrict_device(lmgx_handle *h, int coarse_level, const double *d_res, double *d_b,
                          double *d_f);

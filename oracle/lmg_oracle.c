@@ -323,6 +323,36 @@ void lmgo_sporlx(lmgo *g, int l, const double *const a[7], double *f, const doub
     if (g->jumble > 99999) g->jumble = g->jumble % 99999;
 }
 
+/* code/sporlx2.F:64-146: per iteration 7 sweeps, each over every block with spor in natural
+ * order (io, jo, ko = 1, nip1, nip1*njp1, i.e. the d1 = 1 summation order W,E,S,N,L,R) and a
+ * halo swap after it; the norms are those of the 7th sweep.  jumble is left alone. */
+void lmgo_sporlx2(lmgo *g, int l, const double *const a[7], double *f, const double *b,
+                  double omega, double inout[4], double soltol)
+{
+    double rnpts = 1.0 / g->npts[l];
+    int isweep = 0, niter = 1;
+    double res_norm = inout[0], err_norm = inout[2], var_norm = inout[3];
+    while (isweep < niter && err_norm / var_norm > soltol) {
+        ++isweep;
+        for (int iline = 0; iline < 7; ++iline) {
+            res_norm = 0.0; err_norm = 0.0; var_norm = 0.0;
+            for (int bl = 0; bl < g->nblk; ++bl) {
+                double rs, es, vs;
+                spor_block(g, l, bl, 1, a, f, b, omega, &rs, &es, &vs);
+                res_norm = res_norm + rs;
+                err_norm = err_norm + es;
+                var_norm = var_norm + vs;
+            }
+            lmgo_swap(g, l, f);
+        }
+        err_norm = sqrt(err_norm * rnpts);
+        var_norm = sqrt(var_norm * rnpts);
+        if (var_norm == 0.0) var_norm = 1.0;
+    }
+    res_norm = sqrt(res_norm * rnpts);
+    inout[0] = res_norm; inout[2] = err_norm; inout[3] = var_norm;
+}
+
 /* ------------------------------------------------------------------------------------------
  * Restriction, code/restriction.F:34-65 + code/rest_block.F:61-76 (mult = 2, fac = 1)
  * ------------------------------------------------------------------------------------------ */
@@ -446,7 +476,7 @@ void lmgo_prolong(const lmgo *g, int lf, double *f)
  * Schedule: code/smooth.F:34-100, code/solversc.F:32-35, code/lmg_solve.F:44-160
  * ------------------------------------------------------------------------------------------ */
 static void smooth(lmgo *g, int l, const double *const a[7], double *f, double *b, double *res,
-                   int nsmooth, double omega, double soltol, double nrm[4])
+                   int nsmooth, double omega, double soltol, double nrm[4], int meth_smooth)
 {
     if (l == 0 || l == g->nlev - 1) {
         lmgo_norms(g, l, a, f, b, res, nrm);
@@ -457,7 +487,8 @@ static void smooth(lmgo *g, int l, const double *const a[7], double *f, double *
     int ismooth = 0;
     while (nrm[2] / nrm[3] > soltol && ismooth < nsmooth) {
         ++ismooth;
-        lmgo_sporlx(g, l, a, f, b, omega, nrm, soltol);
+        if (meth_smooth == 2) lmgo_sporlx2(g, l, a, f, b, omega, nrm, soltol);
+        else lmgo_sporlx(g, l, a, f, b, omega, nrm, soltol);
         if (ismooth == nsmooth) lmgo_norms(g, l, a, f, b, res, nrm);
     }
 }
@@ -484,19 +515,20 @@ int lmgo_solve(lmgo *g, const lmgo_params *p, const double *ap, const double *aw
             int goto_fine = (ilev == L - 1) || (nummg[ilev] == 0);
             if (goto_fine) {
                 if (ilev == L - 1) {
-                    smooth(g, ilev, a, f, b, res, p->numcits, p->omega_c, tolc, nrm[ilev]);
+                    smooth(g, ilev, a, f, b, res, p->numcits, p->omega_c, tolc, nrm[ilev],
+                           p->meth_smooth);
                     if (fcycle) gamma = 1;
                 }
                 if (ilev > 0 && L != 1) {
                     ilev = ilev - 1;
                     lmgo_prolong(g, ilev, f);
-                    smooth(g, ilev, a, f, b, res, p->numpost, p->omega, tolc, nrm[ilev]);
+                    smooth(g, ilev, a, f, b, res, p->numpost, p->omega, tolc, nrm[ilev], p->meth_smooth);
                     if (ilev == 0 && fcycle) gamma = 2;
                 }
                 if (ilev == 0) norm_last = nrm[0][0];
                 nummg[ilev] = nummg[ilev] - 1;
             } else {
-                smooth(g, ilev, a, f, b, res, p->numpre, p->omega, tolc, nrm[ilev]);
+                smooth(g, ilev, a, f, b, res, p->numpre, p->omega, tolc, nrm[ilev], p->meth_smooth);
                 ilev = ilev + 1;
                 nummg[ilev] = gamma;
                 lmgo_init_level(g, ilev, f, 0.0);
@@ -573,6 +605,15 @@ void lmgo_global_sum(const lmgo *g, int l, const double *f, double out[3])
         sum = sum + s;
     }
     out[0] = sum; out[1] = mx; out[2] = mn;
+}
+
+void lmgo_p_mean_subtract(const lmgo *g, double *p, double *mean_out)
+{
+    double s[3];
+    lmgo_global_sum(g, 0, p, s);
+    double mean = s[0] / g->npts[0];
+    for (long i = 0; i < g->n1; ++i) p[i] = p[i] - mean;
+    if (mean_out) *mean_out = mean;
 }
 
 /* ------------------------------------------------------------------------------------------

@@ -35,6 +35,7 @@ typedef struct lmgo_params {
     int numpre, numpost, numcits;
     int ncycles;         /* lmg_solve calls with nummgits = 1 */
     double omega, omega_c, tolf;
+    int meth_smooth;     /* 1 = sporlx, 2 = sporlx2 (code/smooth.F:62-73) */
 } lmgo_params;
 
 unsigned long lmgo_sizeof(void);
@@ -53,6 +54,9 @@ void lmgo_norms(const lmgo *g, int l, const double *const a[7], const double *x,
                 double *res, double out[4]);
 void lmgo_sporlx(lmgo *g, int l, const double *const a[7], double *f, const double *b,
                  double omega, double inout[4], double soltol);
+/* code/sporlx2.F:11-161: 7 natural-order sweeps per iteration, no jumble */
+void lmgo_sporlx2(lmgo *g, int l, const double *const a[7], double *f, const double *b,
+                  double omega, double inout[4], double soltol);
 void lmgo_swap(const lmgo *g, int l, double *var);
 void lmgo_restrict(const lmgo *g, int lc, const double *fine, double *coarse);
 void lmgo_prolong(const lmgo *g, int lf, double *f);
@@ -60,6 +64,10 @@ void lmgo_init_level(const lmgo *g, int l, double *f, double value);
 void lmgo_extrap_p(const lmgo *g, double *p);
 void lmgo_p_ref_subtract(const lmgo *g, double *p);
 void lmgo_global_sum(const lmgo *g, int l, const double *f, double out[3]);
+/* Mean-pressure gauge (no reference counterpart; the reference's gauge is p_ref_subtract):
+ * subtract the interior mean of level 1, global_sum / npts_all(1), from every level-1 point,
+ * halos included.  mean_out (may be NULL) receives the mean. */
+void lmgo_p_mean_subtract(const lmgo *g, double *p, double *mean_out);
 
 /* geometry: level-1 interior vertices in, vertices of every level (with ghosts) and the metrics
  * out.  met[m*n2d + .] for m = 0 diu, 1 djv, 2 rdiv, 3 rdju, 4 didjp, 5 rdidjp, 6 rp. */

@@ -34,7 +34,7 @@ def lib():
 class Params(C.Structure):
     _fields_ = [("cycle", C.c_int), ("numpre", C.c_int), ("numpost", C.c_int),
                 ("numcits", C.c_int), ("ncycles", C.c_int), ("omega", C.c_double),
-                ("omega_c", C.c_double), ("tolf", C.c_double)]
+                ("omega_c", C.c_double), ("tolf", C.c_double), ("meth_smooth", C.c_int)]
 
 
 class _G(C.Structure):
@@ -132,9 +132,10 @@ class Oracle:
                            _p(u_int), _p(rho), _p(p), C.c_double(dtime))
         return u
 
-    def solve(self, coeffs, f, b, ncycles, cycle=1, tolf=0.0, res=None):
+    def solve(self, coeffs, f, b, ncycles, cycle=1, tolf=0.0, res=None, meth_smooth=1):
         p = self.problem
-        prm = Params(cycle, p.numpre, p.numpost, p.numcits, ncycles, p.omega, p.omega_c, tolf)
+        prm = Params(cycle, p.numpre, p.numpost, p.numcits, ncycles, p.omega, p.omega_c, tolf,
+                     meth_smooth)
         if res is None:
             res = np.zeros(self.ntot)
         hist = np.zeros((ncycles, self.nlev, 4))
@@ -153,6 +154,13 @@ class Oracle:
         arr = (C.POINTER(C.c_double) * 7)(*[_p(a) for a in coeffs])
         lib().lmgo_sporlx(C.byref(self.g), l, arr, _p(f), _p(b), C.c_double(omega), _p(io),
                           C.c_double(soltol))
+        return io
+
+    def sporlx2(self, l, coeffs, f, b, omega, inout=None, soltol=0.0):
+        io = np.array([0.0, 0.0, 1.0, 1.0]) if inout is None else inout
+        arr = (C.POINTER(C.c_double) * 7)(*[_p(a) for a in coeffs])
+        lib().lmgo_sporlx2(C.byref(self.g), l, arr, _p(f), _p(b), C.c_double(omega), _p(io),
+                           C.c_double(soltol))
         return io
 
     def swap(self, l, var):
@@ -174,6 +182,12 @@ class Oracle:
         out = np.zeros(3)
         lib().lmgo_global_sum(C.byref(self.g), l, _p(f), _p(out))
         return out
+
+    def p_mean_subtract(self, p):
+        """Subtract the level-1 interior mean from every level-1 point; returns the mean."""
+        m = C.c_double()
+        lib().lmgo_p_mean_subtract(C.byref(self.g), _p(p), C.byref(m))
+        return m.value
 
     @property
     def jumble(self):

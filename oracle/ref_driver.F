@@ -16,7 +16,8 @@ C block.dat / x.bin, since only their values matter to the path.
 C
 C Input (stream, little endian; i4 = int32, r8 = float64):
 C   i4  magic(=4711), numbl, nummgl_req, meth_cycle, numpre, numpost,
-C       numcits, ncycles, coeff_src, src_src, do_post, ntime
+C       numcits, ncycles, coeff_src, src_src, do_post, ntime,
+C       meth_smooth (1 = sporlx, 2 = sporlx2; smooth.F:62-73)
 C   r8  relax_pmg, relax_pmgc, tolf, dtime
 C   per block: i4 nijkp1(3), bc(6), bl2bl(6), bl2side(6)
 C   per block: r8 x(2:nip1,2:njp1), y(2:nip1,2:njp1), z(2:nkp1)
@@ -25,6 +26,11 @@ C   src_src=0  : r8 u,v,w,rho,rhonp1 (n1 each)   src_src=1: r8 b (n1)
 C   r8  f (n1)   initial guess on level 1
 C Output: see the write statements at the end; do_post < 0 writes only
 C   i4 4713, ncycles, nummgl; r8 tsolve; r8 hist(4,nummgl,ncycles).
+C   After hist the full output holds the reference's global_sum
+C   (global_sum.F:5-97) of the final f, b and residu on every level,
+C   r8 gsum(3,3,nummgl) = (sum,max,min) x (f,b,residu) x level, and the
+C   per-block results of global_sum_block (global_sum.F:105-147),
+C   r8 gsb(3,3,numbl,nummgl).
 C=======================================================================
 
       program ref_driver
@@ -43,7 +49,7 @@ C=======================================================================
 #include "cb.address.inc"
 
       integer ierr, magic, nreq, ncycles, coeff_src, src_src, do_post
-      integer ntime, itime
+      integer ntime, itime, msmooth
       integer iblk, is, ilev, n1, ntot, n2d, n1d, nxy, nz, icyc
       integer nip1, njp1, nkp1, i, j, ijk, off, lwork, lgeo
       integer ijkse(6)
@@ -60,6 +66,7 @@ C=======================================================================
       real, allocatable :: m2(:,:), dkp(:), dkw(:)
       integer, allocatable :: pnti(:,:)
       real, allocatable :: hist(:,:,:)
+      real, allocatable :: gsum(:,:,:), gsb(:,:,:,:)
       integer iwork(16)
 
       call MPI_INIT(ierr)
@@ -69,7 +76,8 @@ C=======================================================================
       open(unit=10, file=fin, access='stream', form='unformatted',
      &     status='old')
       read(10) magic, numbl, nreq, meth_cycle, numpre, numpost,
-     &     numcits, ncycles, coeff_src, src_src, do_post, ntime
+     &     numcits, ncycles, coeff_src, src_src, do_post, ntime,
+     &     msmooth
       if (magic .ne. 4711) stop 'bad magic'
       read(10) relax_pmg, relax_pmgc, tolf, dtime
 
@@ -81,7 +89,7 @@ C COMMUNICATION AND CONTROL COMMONS (ONE RANK OWNS EVERY BLOCK)
       numbl_total = numbl
       gotrad = .false.
       nummgl = nreq
-      meth_smooth = 1
+      meth_smooth = msmooth
       meth_csolve = 0
       meth_rest = 1
       meth_prol = 1
@@ -269,6 +277,14 @@ C TIMING ONLY (do_post < 0): THE SOLVE TIME AND THE NORMS, NO FIELDS
       write(11) b, f, res
       write(11) hist
 
+C GLOBAL SUMS OF THE FINAL FIELDS, BEFORE THE POST-SOLVE STEPS
+
+      allocate(gsum(3,3,nummgl), gsb(3,3,numbl,nummgl))
+      call drv_gsums(f, gsum(1,1,1), gsb(1,1,1,1))
+      call drv_gsums(b, gsum(1,2,1), gsb(1,2,1,1))
+      call drv_gsums(res, gsum(1,3,1), gsb(1,3,1,1))
+      write(11) gsum, gsb
+
 C POST-SOLVE: ZERO-GRADIENT HALOS + SWAP, THEN REFERENCE-POINT SHIFT
 
       if (do_post .ne. 0) then
@@ -329,4 +345,45 @@ C       (namvar, cb.control.inc:43), which read_3's extr_string scans
       close(11)
 
       call MPI_FINALIZE(ierr)
+      end
+
+
+C=======================================================================
+C global_sum (all blocks) and global_sum_block (each block on its own)
+C of one multi-level field on every level.  gs(1:3, ., ilev) gets
+C (sum, max, min) with a stride of 9 between levels; gsb(1:3, ., iblk,
+C ilev) the block's own results, max and min started afresh per block.
+C=======================================================================
+
+      subroutine drv_gsums(fld, gs, gsb)
+
+      implicit none
+
+#include "dim.inc"
+#include "cb.lgrid.inc"
+#include "cb.lcontrol.inc"
+#include "cb.control.inc"
+
+      real    fld(*), gs(9,*), gsb(9,numbl,*)
+      integer ilev, iblk, p3
+      real    s, tmax, tmin
+
+      do ilev = 1,nummgl
+        call global_sum(ilev,1,fld,gs(1,ilev),gs(2,ilev),gs(3,ilev))
+        do iblk = 1,numbl
+          p3 = mg_off(iblk,ilev) + 1
+          tmax = -1.e30
+          tmin =  1.e30
+          call global_sum_block(fld(p3),mg_nijkp1(1,iblk,ilev),
+     &         mg_nijkp1(2,iblk,ilev),mg_nijkp1(3,iblk,ilev),
+     &         nijkse(1,iblk,ilev),nijkse(2,iblk,ilev),
+     &         nijkse(3,iblk,ilev),nijkse(4,iblk,ilev),
+     &         nijkse(5,iblk,ilev),nijkse(6,iblk,ilev),s,tmax,tmin)
+          gsb(1,iblk,ilev) = s
+          gsb(2,iblk,ilev) = tmax
+          gsb(3,iblk,ilev) = tmin
+        end do
+      end do
+
+      return
       end

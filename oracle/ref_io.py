@@ -27,12 +27,13 @@ def available(gpu: bool = False) -> bool:
 def write_input(path: str, problem, *, ncycles: int, coeffs: Optional[Sequence[np.ndarray]] = None,
                 b1: Optional[np.ndarray] = None, fields: Optional[dict] = None,
                 f0: Optional[np.ndarray] = None, n1: Optional[int] = None, do_post: int = 1,
-                ntime: int = 1, tolf: float = 0.0, cycle: int = 1) -> None:
+                ntime: int = 1, tolf: float = 0.0, cycle: int = 1, meth_smooth: int = 1) -> None:
     p = problem
     with open(path, "wb") as fh:
         w = fh.write
-        w(struct.pack("<12i", 4711, p.nblk, p.nummgl, cycle, p.numpre, p.numpost, p.numcits, ncycles,
-                      0 if coeffs is None else 1, 0 if b1 is None else 1, do_post, ntime))
+        w(struct.pack("<13i", 4711, p.nblk, p.nummgl, cycle, p.numpre, p.numpost, p.numcits, ncycles,
+                      0 if coeffs is None else 1, 0 if b1 is None else 1, do_post, ntime,
+                      meth_smooth))
         w(struct.pack("<4d", p.omega, p.omega_c, tolf, p.dtime))
         for b in p.blocks:
             w(struct.pack("<21i", *b.nijkp1, *b.bc, *b.bl2bl, *b.bl2side))
@@ -92,6 +93,10 @@ def read_output(path: str) -> Dict[str, np.ndarray]:
         out[k] = reals(ntot)
     out["b"], out["f"], out["res"] = reals(ntot), reals(ntot), reals(ntot)
     out["hist"] = reals(4 * L * ncyc).reshape(ncyc, L, 4)
+    # global_sum of (f, b, residu) per level: [level][field][sum, max, min]; global_sum_block of
+    # each block: [level][block][field][sum, max, min]
+    out["gsum"] = reals(9 * L).reshape(L, 3, 3)
+    out["gsum_blk"] = reals(9 * nbl * L).reshape(L, nbl, 3, 3)
     out["p_post"] = reals(n1)
     nvel = int(ints(1)[0])
     if nvel:
@@ -107,16 +112,18 @@ def read_output(path: str) -> Dict[str, np.ndarray]:
 def run(problem, *, ncycles: int, coeffs=None, b1=None, fields=None, f0=None, n1=None,
         do_post: int = 1, ntime: int = 1, tolf: float = 0.0, timeout: float = 3600.0,
         cycle: int = 1, gpu: bool = False, env_extra: Optional[dict] = None,
-        dump: bool = False):
+        dump: bool = False, meth_smooth: int = 1):
     """dump: also write the final p with the reference's file_3 (binary and text, in the
     driver's working directory) and read both back with read_3; the files' bytes are returned
-    as out["file_p_bin"] / out["file_p_txt"]."""
+    as out["file_p_bin"] / out["file_p_txt"].  meth_smooth: the reference's smoother switch
+    (1 = sporlx, 2 = sporlx2).  out["stdout"] is what the driver printed."""
     if not available(gpu):
         raise RuntimeError("reference driver not built: make -C oracle ref ref-gpu")
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         write_input(fin, problem, ncycles=ncycles, coeffs=coeffs, b1=b1, fields=fields, f0=f0,
-                    n1=n1, do_post=do_post, ntime=ntime, tolf=tolf, cycle=cycle)
+                    n1=n1, do_post=do_post, ntime=ntime, tolf=tolf, cycle=cycle,
+                    meth_smooth=meth_smooth)
         env = dict(os.environ)
         env.update(env_extra or {})
         args = [DRIVER_GPU if gpu else DRIVER, fin, fout] + (["dump"] if dump else [])
@@ -125,6 +132,7 @@ def run(problem, *, ncycles: int, coeffs=None, b1=None, fields=None, f0=None, n1
         if r.returncode != 0 or not os.path.exists(fout):
             raise RuntimeError(f"ref_driver failed ({r.returncode}): {r.stdout}\n{r.stderr}")
         out = read_output(fout)
+        out["stdout"] = r.stdout
         if dump:
             for name in ("p_bin", "p_txt"):
                 with open(os.path.join(td, name), "rb") as fh:

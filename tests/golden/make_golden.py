@@ -24,6 +24,12 @@ injected, those arrays) and the reference's outputs.  Cases:
   g10_c4s8_dump_special  the same for a crafted field (no solve): seeded magnitudes 1e-30..1e30
                and the edge cases of G15.8 and of the REAL*4 conversion (zeros, rounding carries,
                3-digit exponents, subnormals, overflow, Inf, NaN)
+  g11_c4s8_spor2  as g2 with the reference's alternative smoother sporlx2 (meth_smooth = 2:
+               7 natural-order SOR sweeps per smoothing iteration, no jumble), 4 V(2,2)
+  g12_c1s8_spor2  the same on 4 blocks (two x-chains), 3 F-cycles
+
+g1-g5, g11 and g12 also hold global_sum (global_sum.F:5-97) of the final f, b and residu on
+every level (gsum) and global_sum_block of each block (gsum_blk), computed by the reference.
 """
 from __future__ import annotations
 
@@ -42,7 +48,7 @@ import oracle as O  # noqa: E402
 import ref_io  # noqa: E402
 
 FULL_KEYS = ("ap", "aw", "ae", "as", "an", "al", "ar", "b", "f", "res", "hist", "p_post",
-             "x", "y", "z", "dkp", "dkw", "npts_all", "nijkp1", "mg_off")
+             "x", "y", "z", "dkp", "dkw", "npts_all", "nijkp1", "mg_off", "gsum", "gsum_blk")
 MET_COLS = {"diu": 1, "djv": 9, "rdiv": 5, "rdju": 12, "didjp": 29, "rdidjp": 30, "rp": 26}
 
 
@@ -132,6 +138,14 @@ def main():
         out = ref_io.run(p, ncycles=ncyc, fields=fields(p), n1=sum(int(np.prod(b.nijkp1)) for b in p.blocks),
                          cycle=cyc)
         save(name, out, dict(cfg=cfg, scale=sc, ncycles=ncyc, cycle=cyc))
+
+    for name, cfg, sc, ncyc, cyc in [("g11_c4s8_spor2", "c4", 8, 4, 1),
+                                     ("g12_c1s8_spor2", "c1", 8, 3, 2)]:
+        p = P.config(cfg, sc)
+        out = ref_io.run(p, ncycles=ncyc, fields=fields(p),
+                         n1=sum(int(np.prod(b.nijkp1)) for b in p.blocks), cycle=cyc,
+                         meth_smooth=2)
+        save(name, out, dict(cfg=cfg, scale=sc, ncycles=ncyc, cycle=cyc, meth_smooth=2))
 
     p = P.config("c4", 8)
     o, a, b1 = ext_inputs(p)

@@ -51,3 +51,34 @@ def test_bad_topology_is_an_error_without_gpu():
                          np.zeros((1, 6), int), np.zeros((1, 6), int), nlev=5)
     with pytest.raises(lmgx.LmgxError, match="levels"):
         lmgx.Solver(topo)
+
+
+def test_array_arguments_are_checked_before_the_library():
+    """A wrong dtype, a CPU tensor where a device array is needed, or a short array raises
+    LmgxError in the binding, before any pointer reaches the library (which cannot check
+    device pointers)."""
+    import numpy as np
+    import torch
+    from fortranutils_amd import lmgx
+    s = object.__new__(lmgx.Solver)          # no handle: nothing may reach the library
+    s.h, s.ntot, s.n1, s.nlev, s.nblk, s.n2d, s.n1d = None, 100, 60, 2, 1, 30, 10
+    ok = np.zeros(100)
+    with pytest.raises(lmgx.LmgxError, match="float64"):
+        s.lmg_solve(np.zeros(100, np.float32), ok, ok)
+    with pytest.raises(lmgx.LmgxError, match="at least 100"):
+        s.lmg_solve(ok, np.zeros(99), ok)
+    with pytest.raises(lmgx.LmgxError, match="contiguous"):
+        s.lmg_solve(ok, ok, np.zeros(200)[::2])
+    with pytest.raises(lmgx.LmgxError, match="CUDA tensor"):
+        s.lmg_solve(torch.zeros(100, dtype=torch.float64), torch.zeros(100), torch.zeros(100))
+    with pytest.raises(lmgx.LmgxError, match="CUDA tensor"):
+        s.p_ref_subtract(torch.zeros(60, dtype=torch.float64))
+    with pytest.raises(lmgx.LmgxError, match="CUDA tensor is required, got list"):
+        s.global_sum(0, [0.0] * 100)
+    with pytest.raises(lmgx.LmgxError, match="7 coefficient"):
+        s.set_operator([ok] * 6)
+    with pytest.raises(lmgx.LmgxError, match="at least 100"):
+        s.set_operator([ok] * 6 + [np.zeros(50)])
+    with pytest.raises(lmgx.LmgxError, match="dkw"):
+        s.uvw_adv(1, np.zeros(60), np.zeros(60), np.zeros(60), np.zeros(60), 0.1,
+                  diu=np.zeros(30), rp=np.zeros(30), dkw=np.zeros(9))

@@ -301,6 +301,41 @@ def test_formed_diagonal_is_the_stored_one(cfg, scale, prec, monkeypatch):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_operator_set_after_a_builder_solve(prec, monkeypatch):
+    """One handle: a solve on the builder's operator (whose level-1 residual forms interior
+    diagonals from the couplings), then lmgx_set_operator with every interior diagonal of
+    level 1 raised by 5%, then a solve.  The second solve must read the new diagonal: it gives
+    bitwise the pressure, residual and history of a fresh handle given the same coefficients,
+    and it differs from the first solve's pressure.  The cycle captured for the builder's
+    operator must not be replayed for the new one."""
+    import torch
+    from fortranutils_amd import lmgx
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg")
+    p, o, a, coeffs, b = setup("c4", 4)
+    kw = dict(max_cycles=4, tolf=0.0, rtol=0.0, smoother=1, precision=prec, numpre=1,
+              numpost=1, omega=1.35, omega_c=1.0, sweeps_per_iter=3)
+    f0 = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    a.lmg_solve(f0, b.clone(), torch.zeros_like(f0), **kw)
+    m = interior_mask(o, 0)
+    new = [c.copy() for c in coeffs]
+    ap1 = new[0][: o.n1]
+    ap1[m] *= 1.05
+    a.set_operator(new)
+    c = lmgx.Solver(lmgx.Topology.from_problem(p))
+    c.set_dirichlet(np.array([bl.dirichlet for bl in p.blocks], np.int32))
+    c.set_operator(new)
+    out = []
+    for s in (a, c):
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b.clone(), res, **kw)
+        out.append((f.cpu().numpy(), hist, res.cpu().numpy()))
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+    assert not np.array_equal(out[0][0], f0.cpu().numpy())
+
+
 @pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c1", 2), ("c0", 1)])
 def test_eight_wavefront_segmented_solve_is_the_same_solve(cfg, scale, monkeypatch):
     """The segmented line solve in 8-wavefront workgroups (lines of at most 8 segments) cuts the

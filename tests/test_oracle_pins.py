@@ -88,3 +88,114 @@ def test_uniform_grid_closed_form():
     # rows without a Dirichlet face sum to zero (A 1 = 0 away from the outflow)
     inner = ap[K, J, 1:-2] - (aw + ae + as_ + an + al + ar)[K, J, 1:-2]
     assert np.max(np.abs(inner)) <= 1e-12 * np.max(ap)
+
+
+def _block_view(o, a, level, bl):
+    n = o.nijkp1[level, bl]
+    off = int(o.off[level, bl])
+    return a[off: off + int(np.prod(n))].reshape(n[2], n[1], n[0])
+
+
+@pytest.mark.parametrize("cfg,scale", [("c4", 8), ("c1", 8), ("c0", 4)])
+def test_rho_levels_linear_field_closed_form(cfg, scale):
+    """A level-1 density linear in the block's own indices, rho = 1 + i/4 + j/2 + k/8, has as
+    8-child mean at coarse cell (I, J, K) the value at the children's centre, i = 2I - 3/2
+    (children 2I-2 and 2I-1), exactly: every value here is a short dyadic rational.  Each
+    coarse level is linear again in its own indices, so every level is checked bit for bit."""
+    p = P.config(cfg, scale)
+    o = O.Oracle(p)
+    rho1 = np.zeros(o.n1)
+    for bl in range(o.nblk):
+        n = o.nijkp1[0, bl]
+        k, j, i = np.meshgrid(np.arange(1, n[2] + 1), np.arange(1, n[1] + 1),
+                              np.arange(1, n[0] + 1), indexing="ij")
+        _block_view(o, rho1, 0, bl)[...] = 1.0 + i / 4.0 + j / 2.0 + k / 8.0
+    rho = o.rho_levels(rho1)
+    for level in range(1, o.nlev):
+        # fine index of the centre of coarse cell I: c_L(I) = c_(L-1)(2I - 3/2), c_0(i) = i,
+        # so c_L(I) = 2^L I - (3/2)(2^L - 1)
+        m = 2 ** level
+        for bl in range(o.nblk):
+            n = o.nijkp1[level, bl]
+            K, J, I = np.meshgrid(np.arange(2, n[2]), np.arange(2, n[1]), np.arange(2, n[0]),
+                                  indexing="ij")
+            c = lambda X: m * X - 1.5 * (m - 1)
+            want = 1.0 + c(I) / 4.0 + c(J) / 2.0 + c(K) / 8.0
+            got = _block_view(o, rho, level, bl)[1:-1, 1:-1, 1:-1]
+            assert np.array_equal(got, want), (level, bl)
+
+
+def _zero_grad_then_swap(o, a, level):
+    """Halos of one level: each physical side in the order 1..6 copies its adjacent interior
+    plane over the whole face (extrap_p_block.F:43-74 semantics), then the interface/periodic
+    halo swap (the oracle's swap, pinned bitwise to swap_sc by the fixtures)."""
+    p = o.problem
+    for bl in range(o.nblk):
+        v = _block_view(o, a, level, bl)
+        for s in range(1, 7):
+            if p.blocks[bl].bc[s - 1] in (P.BCT_INTERFACE, P.BCT_PERIODIC):
+                continue
+            ax = 2 - (s - 1) // 2          # axis of the [k, j, i] view
+            idx = [slice(None)] * 3
+            src = [slice(None)] * 3
+            idx[ax], src[ax] = (0, 1) if s % 2 == 1 else (-1, -2)
+            v[tuple(idx)] = v[tuple(src)]
+    o.swap(level, a)
+
+
+@pytest.mark.parametrize("cfg,scale", [("c4", 8), ("c1", 8)])
+def test_rho_levels_brute_force(cfg, scale):
+    """Random positive level-1 densities over every level: each coarse interior cell is the
+    plain mean of its 2x2x2 children (numpy reshape-mean, to 4 ulp), and the coarse halos are
+    the zero-gradient copy plus swap of that level."""
+    p = P.config(cfg, scale)
+    o = O.Oracle(p)
+    rng = np.random.default_rng(11)
+    rho1 = 1.0 + 6.0 * rng.random(o.n1)
+    rho = o.rho_levels(rho1)
+    # the coarse halos start at 1 (as in the wrapper), which the corner points of two swapped
+    # faces keep: the swap copies whole faces, corners included, in side order
+    want = np.ones(o.ntot)
+    want[: o.n1] = rho1
+    o.swap(0, want)
+    assert np.array_equal(want[: o.n1], rho[: o.n1])
+    for level in range(1, o.nlev):
+        for bl in range(o.nblk):
+            f = _block_view(o, want, level - 1, bl)[1:-1, 1:-1, 1:-1]
+            nz, ny, nx = f.shape
+            mean = f.reshape(nz // 2, 2, ny // 2, 2, nx // 2, 2).mean(axis=(1, 3, 5))
+            _block_view(o, want, level, bl)[1:-1, 1:-1, 1:-1] = mean
+        _zero_grad_then_swap(o, want, level)
+        sl = slice(int(o.off[level, 0]),
+                   int(o.off[level, 0] + np.sum(np.prod(o.nijkp1[level], axis=1))))
+        assert np.allclose(rho[sl], want[sl], rtol=4 * 2.0 ** -52, atol=0.0), level
+        want[sl] = rho[sl]          # the next level from the oracle's values
+
+
+def test_p_mean_subtract_closed_forms():
+    """Mean-pressure gauge: (1) a constant field becomes exactly zero; (2) p = i on the
+    interior of one block (halos anything) has mean (nip1+1)/2 and leaves i - (nip1+1)/2,
+    halos shifted alike; (3) adding a constant changes nothing beyond rounding; (4) the
+    result has zero interior mean."""
+    p = P.config("c4", 8)
+    o = O.Oracle(p)
+    n = o.nijkp1[0]
+    c = np.full(o.n1, 0.75)
+    assert o.p_mean_subtract(c) == 0.75 and not c.any()
+    q = P.config("c0", 4)
+    oq = O.Oracle(q)
+    m = oq.nijkp1[0, 0]
+    lin = np.zeros(oq.n1)
+    _block_view(oq, lin, 0, 0)[...] = np.arange(1, m[0] + 1)[None, None, :]
+    mean = oq.p_mean_subtract(lin)
+    assert mean == (m[0] + 1) / 2.0
+    assert np.array_equal(_block_view(oq, lin, 0, 0)[0, 0, :], np.arange(1, m[0] + 1) - mean)
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(o.n1)
+    a, b = r.copy(), r + 123.25
+    o.p_mean_subtract(a)
+    o.p_mean_subtract(b)
+    assert np.max(np.abs(a - b)) <= 1e-12
+    from helpers import interior_mask
+    msk = interior_mask(o, 0)
+    assert abs(a[msk].mean()) <= 1e-15 * np.abs(r).max()
