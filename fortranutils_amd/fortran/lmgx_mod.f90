@@ -5,14 +5,19 @@ module lmgx_mod
   implicit none
   private
   public :: lmgx_topology, lmgx_params, lmgx_init, lmgx_free, lmgx_levels, lmgx_default_params, &
-            lmgx_set_operator, lmgx_solve, lmgx_uvw_adv, lmgx_last_error_str, lmgx_shared, &
-            LMGX_SMOOTH_EXACT, LMGX_SMOOTH_LINE, LMGX_PREC_FP64, LMGX_PREC_MIXED
+            lmgx_set_operator, lmgx_set_operator_cached, lmgx_counters, lmgx_solve, &
+            lmgx_uvw_adv, lmgx_last_error_str, lmgx_shared, lmgx_force_upload, &
+            LMGX_SMOOTH_EXACT, LMGX_SMOOTH_LINE, LMGX_SMOOTH_SPORLX2, LMGX_PREC_FP64, &
+            LMGX_PREC_MIXED
 
   ! the one handle of a LESter run, created on first use by lmgx_attach (lmg_solve_gpu.F) and
   ! shared by the drop-ins lmg_solve and uvw_adv
   type(c_ptr), save :: lmgx_shared = c_null_ptr
+  ! set by lmgx_coeff_changed (lmg_solve_gpu.F): the next lmg_solve uploads the coefficients
+  ! even when their arrays and sampled fingerprint are those of the last upload
+  integer(c_int), save :: lmgx_force_upload = 0
 
-  integer(c_int), parameter :: LMGX_SMOOTH_EXACT = 0, LMGX_SMOOTH_LINE = 1
+  integer(c_int), parameter :: LMGX_SMOOTH_EXACT = 0, LMGX_SMOOTH_LINE = 1, LMGX_SMOOTH_SPORLX2 = 2
   integer(c_int), parameter :: LMGX_PREC_FP64 = 0, LMGX_PREC_MIXED = 1
 
   type, bind(c) :: lmgx_topology
@@ -57,6 +62,21 @@ module lmgx_mod
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: h
       real(c_double), intent(in) :: ap(*), aw(*), ae(*), as(*), an(*), al(*), ar(*)
+    end function
+
+    integer(c_int) function lmgx_set_operator_cached(h, ap, aw, ae, as, an, al, ar, force, &
+        uploaded) bind(c, name='lmgx_set_operator_cached')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: ap(*), aw(*), ae(*), as(*), an(*), al(*), ar(*)
+      integer(c_int), value :: force
+      integer(c_int), intent(out) :: uploaded
+    end function
+
+    integer(c_int) function lmgx_counters(h, cnt) bind(c, name='lmgx_counters')
+      import :: c_int, c_ptr, c_long
+      type(c_ptr), value :: h
+      integer(c_long), intent(out) :: cnt(4)
     end function
 
     integer(c_int) function lmgx_solve(h, f, b, residu, p, hist, ncyc) bind(c, name='lmgx_solve')

@@ -193,6 +193,23 @@ class Solver:
                                                 for a, q in zip(coeffs, names)])
             _check(lib().lmgx_set_operator_device(self.h, arr))
 
+    def set_operator_cached(self, coeffs: Sequence[np.ndarray], force: bool = False) -> bool:
+        """Host coefficients as :meth:`set_operator`, uploaded only when ``force``, when the
+        arrays are not those of the last upload through this method, or when a sampled
+        fingerprint of them changed; returns whether they were uploaded."""
+        if len(coeffs) != 7:
+            raise LmgxError(f"set_operator_cached: 7 coefficient arrays are required")
+        up = C.c_int()
+        _check(lib().lmgx_set_operator_cached(self.h, *[_hp(a, self.ntot) for a in coeffs],
+                                              int(bool(force)), C.byref(up)))
+        return bool(up.value)
+
+    def counters(self):
+        """(operator uploads, fast-path operator preparations, graph captures, solves)."""
+        out = (C.c_long * 4)()
+        _check(lib().lmgx_counters(self.h, out))
+        return tuple(out)
+
     def build_operator(self, problem, rho1: Optional[np.ndarray] = None, dirichlet=None):
         """Metrics + coefficients of every level on the device from the level-1 vertices."""
         xv = np.concatenate([np.broadcast_to(b.xv[None, :], (b.nijkp1[1] - 1, b.nijkp1[0] - 1)).ravel()

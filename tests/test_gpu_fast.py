@@ -452,3 +452,37 @@ def test_nonsymmetric_operator(prec):
     x = f.cpu().numpy()[: o.n1][m]
     err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
     assert err < 1e-9, err
+
+
+def test_cached_operator_is_uploaded_once():
+    """lmgx_set_operator_cached with the same host arrays uploads once; the fast path prepares
+    its operator and captures its cycle once over three solves; new values in the same arrays
+    (all coefficients doubled) are seen by the fingerprint and uploaded, and the solve then
+    gives half the pressure."""
+    import torch
+    from fortranutils_amd import lmgx
+    p, o, a, coeffs, b = setup("c4", 4)
+    s = lmgx.Solver(lmgx.Topology.from_problem(p))
+    s.set_dirichlet(np.array([bl.dirichlet for bl in p.blocks], np.int32))
+    kw = dict(max_cycles=60, tolf=0.0, rtol=1e-12, smoother=1, precision=1, numpre=1,
+              numpost=1, omega=1.35, omega_c=1.0, sweeps_per_iter=3)
+    host = [c.copy() for c in coeffs]
+    fs = []
+    for q in range(3):
+        assert s.set_operator_cached(host) == (q == 0)
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        s.lmg_solve(f, b, torch.zeros_like(f), **kw)
+        fs.append(f)
+    assert s.counters() == (1, 1, 1, 3)
+    assert torch.equal(fs[0], fs[2])
+    for c in host:
+        c *= 2.0
+    assert s.set_operator_cached(host)
+    f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    s.lmg_solve(f, b, torch.zeros_like(f), **kw)
+    assert s.counters()[:2] == (2, 2)
+    m = torch.from_numpy(interior_mask(o, 0)).cuda()
+    err = (torch.linalg.norm(2.0 * f[: o.n1][m] - fs[0][: o.n1][m]) /
+           torch.linalg.norm(fs[0][: o.n1][m])).item()
+    assert err < 1e-9
+    assert s.set_operator_cached(host, force=True)

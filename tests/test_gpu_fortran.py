@@ -31,19 +31,86 @@ def _fields(p):
     return ref_io.level1_fields(p, dict(u=us, v=vs, w=ws, rho=rho, rhonp1=rho))
 
 
-@pytest.mark.skipif(not (ref_io.available() and ref_io.available(gpu=True)),
-                    reason="reference drivers not built (needs /root/reference at build time)")
-@pytest.mark.parametrize("cfg,scale,ncyc,cycle", [("c4", 8, 4, 1), ("c1", 8, 3, 1),
-                                                  ("c0", 4, 3, 2)])
-def test_dropin_is_bitwise_the_reference(cfg, scale, ncyc, cycle):
+DRIVERS = pytest.mark.skipif(not (ref_io.available() and ref_io.available(gpu=True)),
+                             reason="reference drivers not built (needs /root/reference at "
+                                    "build time)")
+
+
+def _counters(stdout):
+    """The last counters line the drop-in printed (LMGX_VERBOSE=1): uploads, operator
+    preparations, graph captures, solves."""
+    lines = [l for l in stdout.splitlines() if "lmgx counters" in l]
+    assert lines, stdout[-2000:]
+    return [int(x) for x in lines[-1].split(":")[-1].split()]
+
+
+@DRIVERS
+@pytest.mark.parametrize("cfg,scale,ncyc,cycle,meth", [("c4", 8, 4, 1, 1), ("c1", 8, 3, 1, 1),
+                                                       ("c0", 4, 3, 2, 1), ("c4", 8, 3, 1, 2),
+                                                       ("c1", 8, 2, 2, 2)])
+def test_dropin_is_bitwise_the_reference(cfg, scale, ncyc, cycle, meth):
+    """meth = the reference's meth_smooth: 1 sporlx, 2 sporlx2.  The driver calls lmg_solve
+    once per cycle with the same coefficient arrays: the drop-in uploads them once."""
     p = P.config(cfg, scale)
     n1 = sum(int(np.prod(b.nijkp1)) for b in p.blocks)
-    kw = dict(ncycles=ncyc, fields=_fields(p), n1=n1, cycle=cycle, timeout=300)
+    kw = dict(ncycles=ncyc, fields=_fields(p), n1=n1, cycle=cycle, timeout=300, meth_smooth=meth)
     ref = ref_io.run(p, **kw)
-    gpu = ref_io.run(p, gpu=True, **kw)
+    gpu = ref_io.run(p, gpu=True, env_extra={"LMGX_VERBOSE": "1"}, **kw)
+    assert _counters(gpu["stdout"]) == [1, 0, 0, ncyc]
     for k in KEYS:
         assert ref[k].shape == gpu[k].shape, k
         assert np.array_equal(ref[k], gpu[k]), (k, np.abs(ref[k] - gpu[k]).max())
     h0, h1 = ref["hist"], gpu["hist"]
     assert h0.shape == h1.shape
     assert np.max(np.abs(h1 - h0) / np.maximum(np.abs(h0), 1e-300)) < 1e-12
+
+
+@DRIVERS
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("dirichlet", [True, False])
+def test_dropin_fast_path_matches_converged_reference(prec, dirichlet):
+    """LMGX_SMOOTHER=fast (LMGX_PRECISION=mixed: the FP32 V-cycle in FP64 refinement) selected
+    from LESter through the drop-in.  c4 at 1/8 (two blocks across an x-interface, periodic z):
+    with the face-Dirichlet outflow operator, or with the reference's own coeff_p (all Neumann:
+    singular, gauge fixed by p_ref_subtract); the source is sour_p of the seeded velocities
+    with its interior mean removed (compatible with the Neumann operator).  After 40 calls of
+    one cycle each, p after extrap_p and p_ref_subtract matches the reference's lmg_solve run to
+    convergence (200 cycles) within 1e-9 relative L2.  The coefficients are uploaded and the
+    fast path's operator (split copies, coarse levels, coarsest factor) prepared once; one cycle
+    graph serves all 40 calls."""
+    import oracle as O
+    from helpers import interior_mask
+    p = P.config("c4", 8)
+    n1 = sum(int(np.prod(b.nijkp1)) for b in p.blocks)
+    o = O.Oracle(p)
+    geo = o.geometry()
+    fl = _fields(p)
+    b = o.sour(geo, fl["u"], fl["v"], fl["w"], fl["rho"], fl["rhonp1"], p.dtime)[: o.n1]
+    m = interior_mask(o, 0)
+    b[m] -= b[m].mean()
+    kw = dict(b1=b, n1=n1, timeout=300)
+    if dirichlet:
+        dirich = np.array([bl.dirichlet for bl in p.blocks], np.int32)
+        kw["coeffs"] = o.coeff(geo, rho=o.rho_levels(fl["rho"]), dirichlet=dirich)
+    ref = ref_io.run(p, ncycles=200, **kw)
+    env = {"LMGX_SMOOTHER": "fast", "LMGX_PRECISION": prec, "LMGX_VERBOSE": "1"}
+    gpu = ref_io.run(p, ncycles=40, gpu=True, env_extra=env, **kw)
+    pr, pg = ref["p_post"], gpu["p_post"]
+    err = np.linalg.norm(pg - pr) / np.linalg.norm(pr)
+    print(prec, "dirichlet" if dirichlet else "neumann", "rel L2", err,
+          "rnorm", gpu["hist"][[0, -1], 0, 0])
+    assert err <= 1e-9
+    assert _counters(gpu["stdout"]) == [1, 1, 1, 40]
+
+
+@DRIVERS
+def test_dropin_stops_on_a_method_it_does_not_have():
+    """meth_smooth = 3: the reference stops in smooth (smooth.F:77-79), the drop-in stops
+    before any GPU work, both with a message and no output file."""
+    p = P.config("c4", 8)
+    n1 = sum(int(np.prod(b.nijkp1)) for b in p.blocks)
+    kw = dict(ncycles=1, fields=_fields(p), n1=n1, timeout=120, meth_smooth=3)
+    with pytest.raises(RuntimeError, match="Smoothing method No.    3"):
+        ref_io.run(p, **kw)
+    with pytest.raises(RuntimeError, match="smoothing method    3 is not available"):
+        ref_io.run(p, gpu=True, **kw)

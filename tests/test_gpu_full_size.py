@@ -1,0 +1,111 @@
+"""The headline configurations at full size, through the C ABI.
+
+* c3 (16 uneven blocks, 119.5M cells) and c2 (one 512x256x256 block), the benchmark's solve:
+  mixed precision (FP32 V-cycle on binary16 couplings inside FP64 refinement), V(1,1) with 3
+  over-relaxed line sweeps, to a 1e-10 reduction of the residual.  Checked three ways:
+  - the residual b - A p of the returned p, evaluated independently in torch (FP64, 7-point
+    stencil on the natural layout, coefficients as the library returns them), is at most 1e-10
+    of b;
+  - p agrees with the plain FP64 solve of the same schedule (1e-10 reduction) to 1e-9 relative
+    L2 on the interior;
+  - the mixed solve needs at most one cycle more than the FP64 one.
+* c1 (4 blocks, 2.5M cells): the batched one-launch-per-level form and the per-block launch loop
+  give bitwise the same pressure and history, in both precisions.
+"""
+import numpy as np
+import pytest
+
+from fortranutils_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+SCHEDULE = dict(smoother=1, numpre=1, numpost=1, sweeps_per_iter=3, omega=1.35, omega_c=1.0,
+                numcits=20, tolf=0.0)
+
+
+def _problem(cfg):
+    """The benchmark's problem: builder operator (density, face-Dirichlet outflow), device
+    source from seeded velocities (plus the seeded blobs of c2/c3), interior mean removed."""
+    import torch
+    from fortranutils_amd import lmgx
+    p = P.config(cfg)
+    s = lmgx.Solver(lmgx.Topology.from_problem(p))
+    fl = P.device_fields(p)
+    dirich = np.array([b.dirichlet for b in p.blocks], np.int32)
+    rho1 = fl["rho"].cpu().numpy() if p.rho_kind != "uniform" else None
+    s.build_operator(p, rho1=rho1, dirichlet=dirich)
+    b = torch.zeros(s.ntot, dtype=torch.float64, device="cuda")
+    s.source(fl["u"], fl["v"], fl["w"], fl["rho"], fl["rhonp1"], p.dtime, b)
+    if p.blobs:
+        P.device_blobs(p, b[: s.n1])
+    m = P.interior_mask_torch(p)
+    b1 = b[: s.n1]
+    b1[m] -= b1[m].mean()
+    del fl
+    torch.cuda.synchronize()
+    return p, s, b
+
+
+def _residual_ratio(s, coeffs, f, b):
+    """||b - A f|| / ||b|| over the level-1 interior, block by block in torch (f's interface and
+    periodic halos refreshed by the halo swap first; physical halos carry zero couplings)."""
+    import torch
+    s.swap(0, f)
+    num = torch.zeros((), dtype=torch.float64, device="cuda")
+    den = torch.zeros_like(num)
+    I = (slice(1, -1),) * 3
+    for bl in range(s.nblk):
+        n0, n1, n2 = (int(v) for v in s.nijkp1[0, bl])
+        a, sz = int(s.off[0, bl]), n0 * n1 * n2
+        v = lambda t: t[a: a + sz].view(n2, n1, n0)
+        X, B = v(f), v(b)
+        ap, aw, ae, as_, an, al, ar = (v(c) for c in coeffs)
+        r = (B[I] - ap[I] * X[I] + aw[I] * X[1:-1, 1:-1, :-2] + ae[I] * X[1:-1, 1:-1, 2:]
+             + as_[I] * X[1:-1, :-2, 1:-1] + an[I] * X[1:-1, 2:, 1:-1]
+             + al[I] * X[:-2, 1:-1, 1:-1] + ar[I] * X[2:, 1:-1, 1:-1])
+        num += (r * r).sum()
+        den += (B[I] * B[I]).sum()
+    return (num.sqrt() / den.sqrt()).item()
+
+
+@pytest.mark.parametrize("cfg", ["c3", "c2"])
+def test_headline_mixed_solve_full_size(cfg):
+    import torch
+    print(f"{cfg}: building the problem", flush=True)
+    p, s, b = _problem(cfg)
+    coeffs = [torch.from_numpy(a).cuda() for a in s.get_operator()]
+    m = P.interior_mask_torch(p)
+    out = {}
+    for prec in (1, 0):
+        print(f"{cfg}: solve, precision {prec}", flush=True)
+        f = torch.zeros_like(b)
+        res = torch.zeros_like(b)
+        hist = s.lmg_solve(f, b, res, max_cycles=30, rtol=1e-10, precision=prec, **SCHEDULE)
+        ratio = _residual_ratio(s, coeffs, f, b)
+        print(f"{cfg}: precision {prec}: {len(hist)} cycles, |b - A p| / |b| = {ratio:.3e}",
+              flush=True)
+        assert len(hist) < 30
+        assert ratio <= 1e-10
+        out[prec] = (f[: s.n1][m].clone(), len(hist))
+        del f, res
+    (pm, cm), (pd, cd) = out[1], out[0]
+    err = (torch.linalg.norm(pm - pd) / torch.linalg.norm(pd)).item()
+    print(f"{cfg}: |p_mixed - p_fp64| / |p_fp64| = {err:.3e}, cycles {cm} / {cd}")
+    assert err <= 1e-9
+    assert cm <= cd + 1
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_c1_full_size_batched_equals_per_block(prec, monkeypatch):
+    import torch
+    p, s, b = _problem("c1")
+    out = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("LMGX_PER_BLOCK", mode)
+        f = torch.zeros_like(b)
+        res = torch.zeros_like(b)
+        hist = s.lmg_solve(f, b, res, max_cycles=30, rtol=1e-10, precision=prec, **SCHEDULE)
+        out.append((f.cpu().numpy(), hist, res.cpu().numpy()))
+    print("c1 cycles", len(out[0][1]), len(out[1][1]))
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
