@@ -167,6 +167,13 @@ class Solver:
         self.n2d = int(np.sum(self.nijkp1[..., 0].astype(np.int64) * self.nijkp1[..., 1]))
         self.n1d = int(np.sum(self.nijkp1[..., 2]))
 
+    def _end(self, level: int) -> int:
+        """Length of a multi-level array up to the end of ``level`` (the size a single pass on
+        that level needs)."""
+        if not 0 <= int(level) < self.nlev:
+            raise LmgxError(f"level {level} out of range (0..{self.nlev - 1})")
+        return int(self.off[level, -1] + np.prod(self.nijkp1[level, -1]))
+
     def close(self):
         if getattr(self, "h", None):
             lib().lmgx_free(self.h)
@@ -274,7 +281,7 @@ class Solver:
         """Write level ``level`` of the device field ``f`` (multi-level natural layout) in the
         reference's 3-D file format: ``filefrmt`` 0 unformatted (REAL*4 values, or REAL*8 with
         ``real_kind=8``), 1 text."""
-        need = self.n1 if (int(level) == 0 or int(mg_or_sg) == 2) else self.ntot
+        need = self.n1 if int(mg_or_sg) == 2 else self._end(level)
         _check(lib().lmgx_file_3_device(self.h, os.fsencode(filename), int(filefrmt),
                                         _dp(f, need, "f"),
                                         int(level), int(mg_or_sg), int(real_kind)))
@@ -346,29 +353,32 @@ class Solver:
     # ---- single passes (device tensors) -------------------------------------------------
     def norms(self, level, x, b, res):
         out = np.zeros(4)
-        n = self.ntot
+        n = self._end(level)
         _check(lib().lmgx_norms_device(self.h, level, _dp(x, n, "x"), _dp(b, n, "b"),
                                        _dp(res, n, "res"), _hp(out, 4)))
         return out
 
     def sporlx(self, level, f, b, omega):
-        _check(lib().lmgx_sporlx_device(self.h, level, _dp(f, self.ntot, "f"),
-                                        _dp(b, self.ntot, "b"), C.c_double(omega)))
+        n = self._end(level)
+        _check(lib().lmgx_sporlx_device(self.h, level, _dp(f, n, "f"), _dp(b, n, "b"),
+                                        C.c_double(omega)))
 
     def sporlx2(self, level, f, b, omega):
-        _check(lib().lmgx_sporlx2_device(self.h, level, _dp(f, self.ntot, "f"),
-                                         _dp(b, self.ntot, "b"), C.c_double(omega)))
+        n = self._end(level)
+        _check(lib().lmgx_sporlx2_device(self.h, level, _dp(f, n, "f"), _dp(b, n, "b"),
+                                         C.c_double(omega)))
 
     def swap(self, level, var):
-        _check(lib().lmgx_swap_device(self.h, level, _dp(var, self.ntot, "var")))
+        _check(lib().lmgx_swap_device(self.h, level, _dp(var, self._end(level), "var")))
 
     def restrict(self, coarse_level, res, b, f):
-        n = self.ntot
+        n = self._end(coarse_level)
         _check(lib().lmgx_restrict_device(self.h, coarse_level, _dp(res, n, "res"),
                                           _dp(b, n, "b"), _dp(f, n, "f")))
 
     def prolong(self, fine_level, f):
-        _check(lib().lmgx_prolong_device(self.h, fine_level, _dp(f, self.ntot, "f")))
+        _check(lib().lmgx_prolong_device(self.h, fine_level,
+                                         _dp(f, self._end(int(fine_level) + 1), "f")))
 
     def extrap_p(self, p):
         _check(lib().lmgx_extrap_p_device(self.h, _dp(p, self.n1, "p")))
@@ -385,7 +395,8 @@ class Solver:
 
     def global_sum(self, level, f):
         out = np.zeros(3)
-        _check(lib().lmgx_global_sum_device(self.h, level, _dp(f, self.ntot, "f"), _hp(out, 3)))
+        _check(lib().lmgx_global_sum_device(self.h, level, _dp(f, self._end(level), "f"),
+                                            _hp(out, 3)))
         return out
 
     def tile_widths(self):

@@ -62,6 +62,8 @@ def test_array_arguments_are_checked_before_the_library():
     from fortranutils_amd import lmgx
     s = object.__new__(lmgx.Solver)          # no handle: nothing may reach the library
     s.h, s.ntot, s.n1, s.nlev, s.nblk, s.n2d, s.n1d = None, 100, 60, 2, 1, 30, 10
+    s.nijkp1 = np.array([[[5, 4, 3]], [[5, 4, 2]]])
+    s.off = np.array([[0], [60]])
     ok = np.zeros(100)
     with pytest.raises(lmgx.LmgxError, match="float64"):
         s.lmg_solve(np.zeros(100, np.float32), ok, ok)
@@ -75,6 +77,8 @@ def test_array_arguments_are_checked_before_the_library():
         s.p_ref_subtract(torch.zeros(60, dtype=torch.float64))
     with pytest.raises(lmgx.LmgxError, match="CUDA tensor is required, got list"):
         s.global_sum(0, [0.0] * 100)
+    with pytest.raises(lmgx.LmgxError, match="level 2 out of range"):
+        s.global_sum(2, [0.0] * 100)
     with pytest.raises(lmgx.LmgxError, match="7 coefficient"):
         s.set_operator([ok] * 6)
     with pytest.raises(lmgx.LmgxError, match="at least 100"):
