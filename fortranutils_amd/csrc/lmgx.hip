@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <charconv>
+#include <mutex>
 #include <thread>
 #include <cmath>
 #include <cstdio>
@@ -117,6 +118,9 @@ struct lmgx_handle {
     // work counters (lmgx_counters): operator uploads, fast-path operator preparations, cycle
     // graph captures, solves
     long n_uploads = 0, n_prepares = 0, n_captures = 0, n_solves = 0;
+    // device address of the static scratch of the allocation-free entry point on this handle's
+    // GPU (looked up once per GPU by lmgx_init)
+    double *scratch = nullptr;
     // lmgx_set_operator_cached: the host arrays of its last upload and their sampled fingerprint
     const double *cached_src[7] = {nullptr};
     unsigned long long cached_fp = 0;
@@ -438,12 +442,41 @@ extern "C" int lmgx_free(lmgx_handle *h)
     return 0;
 }
 
+// Scratch of lmgx_p_mean_subtract_device_noalloc: the workgroup partials (sum, max, min) of the
+// two-stage level-1 interior sum, then the three final sums and the mean.  A file-scope device
+// array exists on every GPU the library runs on from its load to the end of the process, so
+// that entry point allocates nothing.  In host code `scratch` is only a stand-in for the symbol:
+// kernels get the device address that hipGetSymbolAddress returns for the current GPU.
+#define LMGX_NOALLOC_PARTS 8192
+#define LMGX_MAXDEV 64
+static __device__ __align__(256) double scratch[3 * LMGX_NOALLOC_PARTS + 8];
+static double *g_scratch[LMGX_MAXDEV];
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current GPU, looked up once per GPU
+static int scratch_address(double **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= LMGX_MAXDEV) return fail(LMGX_E_UNSUPPORTED, "device index too large");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(LMGX_E_HIP, "hipGetSymbolAddress returned NULL");
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
 extern "C" int lmgx_init(const lmgx_topology *topo, lmgx_handle **out)
 {
     if (!out) return fail(LMGX_E_ARG, "out is NULL");
     *out = nullptr;
     lmgx_handle *h = new lmgx_handle();
     int rc = build_tables(h, topo);
+    if (rc == 0) rc = scratch_address(&h->scratch);
     if (rc == 0) {
         if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
             rc = fail(LMGX_E_HIP, "hipStreamCreate failed");
@@ -1025,6 +1058,34 @@ extern "C" int lmgx_p_mean_subtract_device(lmgx_handle *h, double *d_p, double *
                           h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (mean_out) memcpy(mean_out, h->h_pinned, sizeof(double));
+    return 0;
+}
+
+extern "C" int lmgx_p_mean_subtract_device_noalloc(lmgx_handle *h, double *d_p, double *d_mean)
+{
+    if (!h || !d_p) return fail(LMGX_E_ARG, "NULL argument");
+    if (!h->scratch) return fail(LMGX_E_STATE, "no scratch address for this handle's GPU");
+    LevelDev &D = h->lev[0];
+    if (D.tiles_int > LMGX_NOALLOC_PARTS)
+        return fail(LMGX_E_UNSUPPORTED,
+                    "level 1 needs " + std::to_string(D.tiles_int) +
+                        " workgroup partials; the static scratch holds " +
+                        std::to_string(LMGX_NOALLOC_PARTS) +
+                        " (use lmgx_p_mean_subtract_device)");
+    // the same kernels as lmgx_p_mean_subtract_device, on the static scratch: every partial
+    // the final sum reads is written first by k_gsum, so nothing needs clearing
+    double *part = h->scratch, *sums = h->scratch + 3 * LMGX_NOALLOC_PARTS;
+    double *mean = d_mean ? d_mean : sums + 3;
+    hipLaunchKernelGGL(k_gsum, dim3(D.tiles_int), dim3(LMGX_WG), 0, h->stream, D.d_lb, h->nblk,
+                       (const double *)d_p, part);
+    hipLaunchKernelGGL(k_gsum_final, dim3(1), dim3(LMGX_WG), 0, h->stream, (const double *)part,
+                       D.tiles_int, sums);
+    hipLaunchKernelGGL(k_mean_of, dim3(1), dim3(64), 0, h->stream, (const double *)sums,
+                       h->npts[0], mean);
+    hipLaunchKernelGGL(k_sub_scalar, dim3(nblocks(h->n1)), dim3(LMGX_WG), 0, h->stream, d_p, h->n1,
+                       (const double *)mean);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 

@@ -393,6 +393,12 @@ class Solver:
         _check(lib().lmgx_p_mean_subtract_device(self.h, _dp(p, self.n1, "p"), C.byref(m)))
         return m.value
 
+    def p_mean_subtract_noalloc(self, p, mean=None):
+        """The mean-pressure gauge through the entry point that allocates nothing (static device
+        scratch); ``mean``: optional float64 CUDA tensor of >= 1 element receiving the mean."""
+        m = None if mean is None else _dp(mean, 1, "mean")
+        _check(lib().lmgx_p_mean_subtract_device_noalloc(self.h, _dp(p, self.n1, "p"), m))
+
     def global_sum(self, level, f):
         out = np.zeros(3)
         _check(lib().lmgx_global_sum_device(self.h, level, _dp(f, self._end(level), "f"),

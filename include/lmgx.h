@@ -144,6 +144,17 @@ int lmgx_global_sum_device(lmgx_handle *h, int level, const double *d_f, double 
  * subtracted.  Device array of n1 (or ntot) doubles. */
 int lmgx_p_mean_subtract_device(lmgx_handle *h, double *d_p, double *mean_out);
 
+/* The same mean-pressure gauge for callers that cannot allocate memory (code under graph
+ * capture, tight serving loops, harnesses that forbid allocation); no counterpart in the
+ * reference.  It calls no hipMalloc/hipFree and no allocating library routine: the workgroup
+ * partials live in a static device array of 8192 partials (about 192 KiB) that every GPU the
+ * library runs on holds until the process ends.  Level 1 may need at most 8192 partials (one per
+ * 64 x 4 column tile of each block's interior), else LMGX_E_UNSUPPORTED.  d_mean: device
+ * pointer receiving the mean, or NULL.  Same result as lmgx_p_mean_subtract_device, bit for
+ * bit.  All calls on one GPU share the array: calls that overlap on one GPU (other host
+ * threads, other handles) race and must be serialised by the caller. */
+int lmgx_p_mean_subtract_device_noalloc(lmgx_handle *h, double *d_p, double *d_mean);
+
 /* Single passes (device arrays, level 0-based), exposed for testing and for callers that drive
  * their own schedule: code/norms.F:5-126, code/sporlx.F:5-203 (one call = 3 sweeps),
  * code/swap_sc0.F:6-293, code/restriction.F:5-68 (+ init_array of the coarse f),
