@@ -9,6 +9,7 @@
 #include <charconv>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
