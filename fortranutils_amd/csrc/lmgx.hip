@@ -104,6 +104,9 @@ struct lmgx_handle {
     long pinned_cap = 0;
     int jumble = 0;
     bool have_op = false;
+    // d_a (natural layout) holds the current operator; false after a density update that wrote
+    // only the fast path's split copy (ensure_natural rebuilds it on demand)
+    bool nat_valid = true;
     // geometry (lmgx_build_operator)
     double *d_met = nullptr;            // [7][n2d]: diu djv rdiv rdju didjp rdidjp rp
     double *d_dkp = nullptr, *d_dkw = nullptr;
@@ -540,6 +543,7 @@ static int ensure_operator_storage(lmgx_handle *h)
 }
 
 static int fast_operator_changed(lmgx_handle *h);
+static int ensure_natural(lmgx_handle *h);
 
 extern "C" int lmgx_set_operator(lmgx_handle *h, const double *ap, const double *aw,
                                  const double *ae, const double *as, const double *an,
@@ -555,6 +559,7 @@ extern "C" int lmgx_set_operator(lmgx_handle *h, const double *ap, const double 
                               h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_op = true;
+    h->nat_valid = true;
     h->aT_valid = false;
     h->op_builder = false;
     h->cached_valid = false;
@@ -618,6 +623,7 @@ extern "C" int lmgx_set_operator_device(lmgx_handle *h, const double *const d_co
                               hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_op = true;
+    h->nat_valid = true;
     h->aT_valid = false;
     h->op_builder = false;
     h->cached_valid = false;
@@ -631,6 +637,7 @@ extern "C" int lmgx_get_operator(const lmgx_handle *h, double *ap, double *aw, d
                                  double *as, double *an, double *al, double *ar)
 {
     if (!h || !h->have_op) return fail(LMGX_E_STATE, "no operator");
+    CHK(ensure_natural(const_cast<lmgx_handle *>(h)));
     double *dst[7] = {ap, aw, ae, as, an, al, ar};
     for (int q = 0; q < 7; ++q)
         if (dst[q])
@@ -889,6 +896,7 @@ extern "C" int lmgx_solve_device(lmgx_handle *h, double *d_f, double *d_b, doubl
     int nc = 0;
     CHK(ensure_hist(h, (long)std::max(1, prm->max_cycles) * h->nlev * 4 + 64));
     ++h->n_solves;
+    if (prm->smoother != LMGX_SMOOTH_LINE) CHK(ensure_natural(h));
     int rc = prm->smoother != LMGX_SMOOTH_LINE ? solve_exact(h, d_f, d_b, d_res, prm, history, &nc)
                                                : solve_fast(h, d_f, d_b, d_res, prm, history, &nc);
     if (ncycles_out) *ncycles_out = nc;
@@ -940,6 +948,7 @@ extern "C" int lmgx_norms_device(lmgx_handle *h, int level, const double *d_x, c
 {
     CHK(check_level(h, level));
     if (!h->have_op) return fail(LMGX_E_STATE, "no operator");
+    CHK(ensure_natural(h));
     CHK(ensure_hist(h, 64));
     CHK(norms_exact(h, level, d_x, d_b, d_res, h->d_scalar));
     double tmp[4];
@@ -953,6 +962,7 @@ extern "C" int lmgx_sporlx_device(lmgx_handle *h, int level, double *d_f, const 
 {
     CHK(check_level(h, level));
     if (!h->have_op) return fail(LMGX_E_STATE, "no operator");
+    CHK(ensure_natural(h));
     CHK(sporlx_exact(h, level, d_f, d_b, omega));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
@@ -963,6 +973,7 @@ extern "C" int lmgx_sporlx2_device(lmgx_handle *h, int level, double *d_f, const
 {
     CHK(check_level(h, level));
     if (!h->have_op) return fail(LMGX_E_STATE, "no operator");
+    CHK(ensure_natural(h));
     CHK(sporlx2_exact(h, level, d_f, d_b, omega));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;

@@ -416,6 +416,12 @@ class Solver:
         _check(lib().lmgx_fused_halos(self.h, _ip(out)))
         return out.reshape(2, MAXL)[:, : self.nlev]
 
+    def coarsest_info(self):
+        """(exact solve, banded factor, half-bandwidth, components) of the coarsest level."""
+        out = np.zeros(4, np.int32)
+        _check(lib().lmgx_coarsest_info(self.h, _ip(out)))
+        return tuple(int(v) for v in out)
+
     def coef16_levels(self):
         """Per level: 1 where the FP32 line relaxation reads binary16 coefficients."""
         out = np.zeros(MAXL, np.int32)

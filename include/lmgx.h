@@ -270,6 +270,14 @@ int lmgx_fused_halos(const lmgx_handle *h, int *out);
  * is set. */
 int lmgx_coef16_levels(const lmgx_handle *h, int *out);
 
+/* The fast path's coarsest-level solve: out[0] = 1 when it is the exact solve (the coarsest
+ * operator factored, its inverse applied per cycle), out[1] = 1 when the factor and inverse came
+ * from the banded Cholesky (components in reverse Cuthill-McKee order, half-bandwidth out[2] at
+ * most 63) rather than dense rocSOLVER, out[3] = connected components of blocks.  Zeros (and
+ * out[2] = -1) before the first fast solve.  LMGX_COARSE_DENSE=1 in the environment at the
+ * handle's first fast solve keeps the dense path. */
+int lmgx_coarsest_info(const lmgx_handle *h, int *out);
+
 /* Kernel timing of the last solve: total duration (ms), count and updated cells of the dominant
  * smoother kernel, measured with HIP events on the handle's stream.  The fast path times the
  * finest level's plain half-sweeps (not the two zero-start half-sweeps that open each cycle's

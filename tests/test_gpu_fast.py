@@ -173,6 +173,38 @@ def test_coarsest_direct_solve_by_components():
     assert cyc[0] >= 3 * cyc[1]
 
 
+@pytest.mark.parametrize("cfg,scale", [("c3", 8), ("c3", 2), ("c1", 1), ("c2", 2)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_banded_coarsest_matches_dense(cfg, scale, prec, monkeypatch):
+    """The coarsest level's banded Cholesky and inverse (components in reverse Cuthill-McKee
+    order) against the dense rocSOLVER factor and rocBLAS inverse (LMGX_COARSE_DENSE=1): on
+    fresh handles with the line kernels pinned, both reach a 1e-11 reduction in the same number
+    of cycles (one more allowed) with pressures within 1e-10 relative, and the banded path is
+    the one taken wherever the half-bandwidth is at most 63."""
+    import torch
+    from fortranutils_amd import lmgx
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg" if prec else "64")
+    out = {}
+    for dense in ("1", "0"):
+        monkeypatch.setenv("LMGX_COARSE_DENSE", dense)
+        p, o, s, coeffs, b = setup(cfg, scale)
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        hist = s.lmg_solve(f, b, torch.zeros_like(f), max_cycles=60, tolf=0.0, rtol=1e-11,
+                           smoother=1, precision=prec, numpre=1, numpost=1, omega=1.35,
+                           omega_c=1.0, sweeps_per_iter=3)
+        info = s.coarsest_info()
+        print(cfg, scale, prec, "dense" if dense == "1" else "banded", len(hist), "cycles", info)
+        assert info[0] == 1
+        assert info[1] == (dense == "0" and info[2] <= 63)
+        out[dense] = (f[: o.n1].clone(), len(hist), info)
+    (fd, cd, _), (fb, cb, ib) = out["1"], out["0"]
+    assert abs(cb - cd) <= 1
+    err = (torch.linalg.norm(fb - fd) / torch.linalg.norm(fd)).item()
+    assert err <= 1e-10, err
+    if cfg == "c3":
+        assert ib[1] == 1 and ib[3] == 7
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_new_operator_between_solves(prec):
     """A second operator on the same handle (here 2A) must not reuse the first one's captured
