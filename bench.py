@@ -278,6 +278,34 @@ def x_chains(p):
     return chains
 
 
+def host_cores():
+    """Cores this process may run on (the host's share given to it)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count()
+
+
+def exact_path_rate(s, b, prob):
+    """Seconds per V-cycle of the reference's own schedule (FP64, ``prob``'s V(numpre, numpost),
+    lexicographic point SOR, ``numcits`` iterations on the coarsest level) on the GPU through
+    the bitwise exact path: the cycle the CPU baseline times, on the whole workload."""
+    import torch
+    f = torch.zeros_like(b)
+    res = torch.zeros_like(b)
+    b = b.clone()                               # the exact path writes coarse-level sources
+    kw = dict(tolf=0.0, rtol=0.0, max_cycles=1, smoother=0, precision=0, numpre=prob.numpre,
+              numpost=prob.numpost, numcits=prob.numcits, omega=prob.omega,
+              omega_c=prob.omega_c)
+    s.lmg_solve(f, b, res, **kw)                # first call builds the natural-layout operator
+    f.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.lmg_solve(f, b, res, **kw)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
 def cpu_baseline_all_chains(args):
     """The reference Fortran (oracle/_ref/ref_driver) on the whole workload: every x-chain in its
     own single-rank process, all at once, one core each.  The chains share no halo, so this is
@@ -310,7 +338,8 @@ def cpu_baseline_all_chains(args):
               f"reference processes (largest chain {100 * big:.0f}% of the cells); the "
               f"reference's own coeff_p, FP64 V(2,2) point SOR, {ncyc} cycle(s)")
     return dict(value=sum(cells) * ncyc / tmax / 1e6, unit="Mcell*V-cycles/s", cores=len(chains),
-                ranks=len(chains), kind="reference", sample=sample, seconds=tmax,
+                ranks=len(chains), host_cores=host_cores(), kind="reference", sample=sample,
+                seconds=tmax,
                 per_process=[dict(cells=c, seconds=t) for c, (t, _) in zip(cells, out)])
 
 
@@ -344,8 +373,8 @@ def cpu_baseline(args):
             out = ref_io.run(q, ncycles=args.cpu_cycles, coeffs=a, b1=bb[: o.n1], n1=o.n1,
                              do_post=0, timeout=600)
             return dict(value=cells * args.cpu_cycles / out["tsolve"] / 1e6,
-                        unit="Mcell*V-cycles/s", cores=1, ranks=1, kind="reference",
-                        sample=sample, seconds=out["tsolve"])
+                        unit="Mcell*V-cycles/s", cores=1, ranks=1, host_cores=host_cores(),
+                        kind="reference", sample=sample, seconds=out["tsolve"])
         except Exception as e:  # the reference build may not run on this host
             print("reference CPU baseline unavailable:", str(e)[:200], file=sys.stderr)
     f = np.zeros(o.ntot)
@@ -353,7 +382,7 @@ def cpu_baseline(args):
     o.solve(a, f, bb, args.cpu_cycles)
     dt = time.perf_counter() - t0
     return dict(value=cells * args.cpu_cycles / dt / 1e6, unit="Mcell*V-cycles/s", cores=1,
-                ranks=1, kind="port", sample=sample, seconds=dt)
+                ranks=1, host_cores=host_cores(), kind="port", sample=sample, seconds=dt)
 
 
 def traffic_from_profiles(workload, kernel):
@@ -553,10 +582,15 @@ def main():
         words = 9 if args.omega == 1.0 else 10
         tm = int(s.tile_widths()[1 if word == 4 else 0][0])
         h16 = word == 4 and tm in (8, 16) and bool(s.coef16_levels()[0])
-        bytes_per_cell = words * word - (6 * word - 12 if h16 else 0)
-        per_launch_bytes = (cells / max(nl, 1)) * bytes_per_cell
+        # algorithmic bytes of an SOR sweep, 10 words per updated cell (7 coefficients, b, x read
+        # and written); the kernel moves fewer: binary16 couplings, an from the next row's as
+        alg_per_cell = 10 * word
+        moved_per_cell = words * word - (6 * word - 12 if h16 else 0)
         avg_s = ms / max(nl, 1) / 1e3
+        cells_launch = cells / max(nl, 1)
+        per_launch_bytes = cells_launch * alg_per_cell
         achieved = per_launch_bytes / avg_s
+        moved_bytes = cells_launch * moved_per_cell
         rl = "true" if args.omega != 1.0 else "false"
         fp = "float" if word == 4 else "double"
         fz = "true" if s.fused_halos()[1 if word == 4 else 0][0] else "false"
@@ -567,18 +601,24 @@ def main():
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=achieved / PEAK_HBM, traffic=tr,
                     kernel=key, avg_launch_us=1e6 * avg_s, launches=nl,
-                    bytes_per_launch=per_launch_bytes, bytes_per_cell=bytes_per_cell,
-                    coef16=h16, gcell_updates_per_s=cells / max(nl, 1) / avg_s / 1e9,
-                    cells_per_launch=cells / max(nl, 1))
-        print(f"# smoother: {nl} launches, avg {1e6 * avg_s:.1f} us, {achieved / 1e9:.0f} GB/s, "
-              f"tile widths fp64/fp32 {s.tile_widths().tolist()}", file=sys.stderr, flush=True)
+                    bytes_per_launch=per_launch_bytes, bytes_per_cell=alg_per_cell,
+                    moved_bytes_per_cell=moved_per_cell, moved_bytes_per_launch=moved_bytes,
+                    achieved_moved=moved_bytes / avg_s / 1e9,
+                    frac_moved=moved_bytes / avg_s / PEAK_HBM,
+                    coef16=h16, gcell_updates_per_s=cells_launch / avg_s / 1e9,
+                    cells_per_launch=cells_launch)
+        print(f"# smoother: {nl} launches, avg {1e6 * avg_s:.1f} us, {achieved / 1e9:.0f} GB/s "
+              f"at {alg_per_cell} algorithmic B per cell ({roof['frac']:.3f} of 8 TB/s), "
+              f"{roof['achieved_moved']:.0f} GB/s at the {moved_per_cell} B it moves "
+              f"({roof['frac_moved']:.3f}); tile widths fp64/fp32 {s.tile_widths().tolist()}",
+              file=sys.stderr, flush=True)
         if not args.no_stream_ceiling:
             sc = stream_ceiling()
             roof["stream_ceiling"] = sc
-            roof["frac_of_stream"] = achieved / 1e9 / max(sc.values())
+            roof["frac_of_stream"] = roof["achieved_moved"] / max(sc.values())
             print(f"# streaming on this GPU: 9-read/1-write {sc['sum9_GBps']:.0f} GB/s, copy "
-                  f"{sc['copy_GBps']:.0f} GB/s; smoother at {roof['frac_of_stream']:.2f} of the "
-                  f"faster", file=sys.stderr, flush=True)
+                  f"{sc['copy_GBps']:.0f} GB/s; smoother's moved bytes at "
+                  f"{roof['frac_of_stream']:.2f} of the faster", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -593,8 +633,17 @@ def main():
             except Exception as e:
                 print("cpu baseline failed:", e, file=sys.stderr)
         if cpu:
-            print(f"# CPU baseline ({cpu['kind']}, {cpu['cores']} cores): {cpu['value']:.2f} "
-                  f"{cpu['unit']}", file=sys.stderr, flush=True)
+            print(f"# CPU baseline ({cpu['kind']}, {cpu['cores']} of {cpu['host_cores']} host "
+                  f"cores): {cpu['value']:.2f} {cpu['unit']}", file=sys.stderr, flush=True)
+            # the same cycle (the reference's V(2,2) point-SOR schedule) on the GPU, bitwise the
+            # reference's arithmetic: the like-for-like rate next to the CPU's
+            dt_ex = exact_path_rate(s, b, p)
+            cpu["gpu_same_cycle"] = dict(value=n1cells / dt_ex / 1e6, unit="Mcell*V-cycles/s",
+                                         seconds_per_cycle=dt_ex,
+                                         ratio_to_cpu=n1cells / dt_ex / 1e6 / cpu["value"])
+            print(f"# the reference's own cycle on the GPU (exact path): {n1cells / dt_ex / 1e6:.1f}"
+                  f" Mcell*V-cycles/s, {cpu['gpu_same_cycle']['ratio_to_cpu']:.0f}x the CPU's",
+                  file=sys.stderr, flush=True)
 
     # the whole solve against the HBM roofline: algorithmic bytes of the schedule / time
     sb = solve_bytes(s, args, args.mode, cycles / args.steps) if args.mode != "exact" else None

@@ -122,6 +122,9 @@ struct lmgx_handle {
     // work counters (lmgx_counters): operator uploads, fast-path operator preparations, cycle
     // graph captures, solves
     long n_uploads = 0, n_prepares = 0, n_captures = 0, n_solves = 0;
+    // line-kernel choices given by lmgx_set_tuning: [precision][kernel, halo stores][level]
+    int tune_req[2][2][LMGX_MAXLEV] = {};
+    bool tune_have[2] = {false, false};
     // device address of the static scratch of the allocation-free entry point on this handle's
     // GPU (looked up once per GPU by lmgx_init)
     double *scratch = nullptr;

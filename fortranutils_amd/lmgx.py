@@ -410,6 +410,20 @@ class Solver:
         _check(lib().lmgx_tile_widths(self.h, _ip(out)))
         return out.reshape(2, MAXL)[:, : self.nlev]
 
+    def get_tuning(self):
+        """The autotuner's choices [precision][kernel, halo stores][level] (zeros: not tuned)."""
+        out = np.zeros(4 * MAXL, np.int32)
+        _check(lib().lmgx_get_tuning(self.h, _ip(out)))
+        return out.reshape(2, 2, MAXL)[:, :, : self.nlev].copy()
+
+    def set_tuning(self, table):
+        """Install a table from :meth:`get_tuning` (same shape) for the next fast solves."""
+        t = np.zeros((2, 2, MAXL), np.int32)
+        a = np.asarray(table, np.int32)
+        t[:, :, : a.shape[2]] = a
+        t = np.ascontiguousarray(t).ravel()
+        _check(lib().lmgx_set_tuning(self.h, _ip(t)))
+
     def fused_halos(self):
         """[precision][level]: 1 where the line kernel stores the face-interior halos itself."""
         out = np.zeros(2 * MAXL, np.int32)

@@ -251,6 +251,19 @@ int lmgx_rho_adjoint_device(lmgx_handle *h, const double *d_lambda, const double
  * for levels that do not exist or before the first fast solve. */
 int lmgx_tile_widths(const lmgx_handle *h, int *out);
 
+/* The line-kernel autotuner's choices, to reuse them (no counterpart in the reference):
+ * out[2][2][LMGX_MAXL] = [precision 0 FP64 / 1 FP32][0: line kernel as lmgx_tile_widths, 1: halo
+ * stores as lmgx_fused_halos][level]; zeros for a precision not tuned yet.
+ * lmgx_set_tuning installs such a table: every precision with a nonzero kernel entry is not timed
+ * at its next fast solve but takes these choices (before a handle's first solve, or later, for
+ * the next solve).  The segmented solve and the halo-store choice change the rounding, so
+ * handles or processes that must give bitwise the same solve share one table; the autotuner,
+ * which times the candidates, can choose differently from run to run.  A choice that a level
+ * cannot run (the segmented solve on a non-symmetric level or on lines it does not hold, halo
+ * stores where the topology allows none) makes that solve return LMGX_E_ARG. */
+int lmgx_get_tuning(const lmgx_handle *h, int *out);
+int lmgx_set_tuning(lmgx_handle *h, const int *in);
+
 /* Levels whose face-interior halos the fast path's line kernels store themselves instead of a
  * swap launch after each half-sweep: out[precision][LMGX_MAXL] like lmgx_tile_widths, 1 = stored
  * by the line kernel, 0 = swapped.  A level qualifies when every swapped face is an i- or k-face

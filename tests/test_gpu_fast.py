@@ -518,3 +518,37 @@ def test_cached_operator_is_uploaded_once():
            torch.linalg.norm(fs[0][: o.n1][m])).item()
     assert err < 1e-9
     assert s.set_operator_cached(host, force=True)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_tuning_table_reproduces_a_solve(prec):
+    """Two handles on c3/4: the second takes the first's autotuner table (lmgx_set_tuning) before
+    its first solve, is not timed, and gives bitwise the first's pressure and history; a line
+    kernel that a level cannot run (the segmented solve in FP64) is refused."""
+    import torch
+    from fortranutils_amd import lmgx
+    kw = dict(max_cycles=5, tolf=0.0, rtol=0.0, smoother=1, precision=prec, numpre=1, numpost=1,
+              omega=1.35, omega_c=1.0, sweeps_per_iter=3)
+    out = []
+    table = None
+    for q in range(2):
+        p, o, s, coeffs, b = setup("c3", 4)
+        if table is not None:
+            s.set_tuning(table)
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        hist = s.lmg_solve(f, b, torch.zeros_like(f), **kw)
+        out.append((f.cpu().numpy(), hist))
+        if table is None:
+            table = s.get_tuning()
+            assert (table[0, 0] > 0).all() and ((table[1, 0] > 0).all() == (prec == 1))
+        else:
+            assert np.array_equal(s.get_tuning(), table)
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+    bad = table.copy()
+    bad[0, 0, 0] = 16
+    p, o, s, coeffs, b = setup("c3", 4)
+    s.set_tuning(bad)
+    f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    with pytest.raises(lmgx.LmgxError, match="cannot run line kernel 16"):
+        s.lmg_solve(f, b, torch.zeros_like(f), **kw)
