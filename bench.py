@@ -493,11 +493,14 @@ def traffic(d):
         word = "float" if mode == "mixed" else "double"
         fetch_rows = list(_counter_rows(glob.glob(os.path.join(sub, "**", "*counter_collection.csv"),
                                                   recursive=True)[0]))
-        # every finest-level line-kernel instance (tile width, relaxation) separately
+        # every finest-level line-kernel instance (tile width, relaxation) separately, the
+        # level-1 FP64 residual passes and the mixed-precision correction
         names = sorted({n.replace("void ", "").split("(")[0] for n, _ in fetch_rows
                         if n.replace("void ", "").startswith((f"k_zebra_line<{word}, 1,",
                                                               f"k_zebra_seg<{word}, 1,",
-                                                              f"k_zebra_seg8<{word}, 1,"))})
+                                                              f"k_zebra_seg8<{word}, 1,",
+                                                              "k_resid_cols<double, float, true",
+                                                              "k_correct"))})
         ent = {}
         for kname in names:
             pick = lambda name, k=kname: name.replace("void ", "").split("(")[0] == k

@@ -252,6 +252,33 @@ def test_line_kernels_converge(kernel, cfg, scale, monkeypatch):
     assert err < 1e-9, err
 
 
+@pytest.mark.parametrize("kernel", ["64", "128", "seg8"])
+@pytest.mark.parametrize("cfg,scale", [("c1", 4), ("c4", 4)])
+def test_fp64_line_kernels_converge(kernel, cfg, scale, monkeypatch):
+    """The FP64 fast path with each of its line kernels pinned (two-ended with 64 or 128 lines per
+    wavefront row; segmented in 8-wavefront workgroups, whose coarse levels start from zero in
+    their first sweep), over-relaxed: scipy's direct solve to 1e-9."""
+    import torch
+    import scipy.sparse.linalg as spla
+    monkeypatch.setenv("LMGX_LINE_KERNEL", kernel)
+    p, o, s, coeffs, b = setup(cfg, scale)
+    A, idx = assemble_level1(o, coeffs)
+    bh = b.cpu().numpy()[: o.n1]
+    m = interior_mask(o, 0)
+    x_ref = spla.spsolve(A.tocsc(), bh[m])
+    f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    res = torch.zeros_like(f)
+    hist = s.lmg_solve(f, b, res, max_cycles=60, tolf=0.0, rtol=1e-11, smoother=1, precision=0,
+                       numpre=1, numpost=1, omega=1.3, omega_c=1.0, sweeps_per_iter=2)
+    tw = s.tile_widths()
+    print(cfg, kernel, "cycles", len(hist), "widths", tw.tolist())
+    assert (tw[0][:-1] == (8 if kernel == "seg8" else int(kernel))).all()
+    assert len(hist) < 60
+    x = f.cpu().numpy()[: o.n1][m]
+    err = np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref)
+    assert err < 1e-9, err
+
+
 @pytest.mark.parametrize("cfg,scale", [("c1", 2), ("c3", 4)])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_per_block_launches_agree(cfg, scale, prec, monkeypatch):
@@ -273,7 +300,8 @@ def test_per_block_launches_agree(cfg, scale, prec, monkeypatch):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-@pytest.mark.parametrize("prec,kernel", [(1, "seg"), (1, "seg8"), (1, "128"), (0, "64")])
+@pytest.mark.parametrize("prec,kernel", [(1, "seg"), (1, "seg8"), (1, "128"), (0, "64"),
+                                         (0, "seg8")])
 @pytest.mark.parametrize("cfg,scale", [("c1", 2), ("c3", 4), ("c0", 1)])
 def test_fused_halo_stores_match_swaps(cfg, scale, prec, kernel, monkeypatch):
     """The line kernels (segmented, two-ended) storing the i- and k-face halos themselves give
