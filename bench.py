@@ -92,31 +92,32 @@ def copy_calib():
     print(json.dumps({"copy_calib_bytes": 4 * n, "dispatches": 3}))
 
 
+# read/write mixes of the streaming probe: pure reads, the level-1 FP64 residual's (40 B read,
+# 4 B written per cell), 9:1, the finest line relaxation's (24 B read, 4 B written) and a copy
+STREAM_MIXES = ((10, 0), (10, 1), (9, 1), (6, 1), (1, 1))
+
+
 def stream_ceiling(reps=5):
-    """What plain streaming kernels reach on this GPU, as the practical ceiling next to the
-    published 8 TB/s: torch's column sum of a (9, n) FP32 tensor (9 words read, 1 written per
-    element, the smoother's word count) and a device copy, n = 62.9M (2.5 GB touched), best of
-    ``reps`` launches timed with events on torch's stream."""
-    import torch
-    n = 60 << 20
-    a = torch.ones((9, n), dtype=torch.float32, device="cuda")
-    y = torch.empty(n, dtype=torch.float32, device="cuda")
-    c = torch.empty_like(a)
+    """What plain streaming kernels reach on this GPU, as the practical ceilings next to the
+    published 8 TB/s: fortranutils_amd/libstreamprobe.so reads R and writes W arrays of 16-byte
+    elements (512 MiB each) in one grid-stride pass, best of ``reps`` timed launches, for each
+    mix of STREAM_MIXES.  HBM3E loses bandwidth when a read stream carries a small share of
+    writes, so a kernel is compared with the ceiling of its own read/write mix."""
+    import ctypes as C
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortranutils_amd",
+                        "libstreamprobe.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: python -c 'import __graft_entry__ as g; g.build()'")
+    lib = C.CDLL(path)
+    lib.lmgx_stream_probe.argtypes = [C.c_int, C.c_int, C.c_long, C.c_int,
+                                      C.POINTER(C.c_double)]
     out = {}
-    for name, fn, nbytes in (("sum9_GBps", lambda: torch.sum(a, dim=0, out=y), 40 * n),
-                             ("copy_GBps", lambda: c.copy_(a), 72 * n)):
-        fn()
-        best = float("inf")
-        for _ in range(reps):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-            fn()
-            e1.record()
-            e1.synchronize()
-            best = min(best, e0.elapsed_time(e1) * 1e-3)
-        out[name] = nbytes / best / 1e9
-    del a, y, c
-    torch.cuda.empty_cache()
+    for r, w in STREAM_MIXES:
+        v = C.c_double()
+        rc = lib.lmgx_stream_probe(r, w, 32 << 20, reps, C.byref(v))
+        if rc:
+            raise RuntimeError(f"stream probe R{r} W{w} failed ({rc})")
+        out[f"r{r}w{w}_GBps"] = v.value
     return out
 
 
@@ -618,10 +619,14 @@ def main():
         if not args.no_stream_ceiling:
             sc = stream_ceiling()
             roof["stream_ceiling"] = sc
-            roof["frac_of_stream"] = roof["achieved_moved"] / max(sc.values())
-            print(f"# streaming on this GPU: 9-read/1-write {sc['sum9_GBps']:.0f} GB/s, copy "
-                  f"{sc['copy_GBps']:.0f} GB/s; smoother's moved bytes at "
-                  f"{roof['frac_of_stream']:.2f} of the faster", file=sys.stderr, flush=True)
+            # the kernel's own mix: b, the other colour's x, the old x and the couplings read,
+            # x written (6:1 with binary16 couplings, 9:1 without)
+            mix = "r6w1_GBps" if h16 else "r9w1_GBps"
+            roof["frac_of_stream"] = roof["achieved_moved"] / sc[mix]
+            print("# streaming on this GPU (reads:writes): " +
+                  ", ".join(f"{k[1:-5].replace('w', ':')} {v:.0f} GB/s" for k, v in sc.items()) +
+                  f"; smoother's moved bytes at {roof['frac_of_stream']:.2f} of its own mix's",
+                  file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:

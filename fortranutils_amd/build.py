@@ -32,6 +32,20 @@ def build_lib(verbose: bool = False) -> str:
     return out
 
 
+def build_stream_probe(verbose: bool = False) -> str:
+    """libstreamprobe.so: the benchmark's read/write-mix bandwidth probe (not part of liblmgx)."""
+    src = os.path.join(HERE, "csrc", "stream_probe.hip")
+    out = os.path.join(HERE, "libstreamprobe.so")
+    tmp = out + ".tmp"
+    r = subprocess.run([HIPCC, *HIP_FLAGS, "-o", tmp, src], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_fortran(verbose: bool = False) -> str | None:
     """Compile the Fortran interface module (fortran/lmgx_mod.f90) when flang is present; the
     drop-in lmg_solve (fortran/lmg_solve_gpu.F) is compiled inside LESter's build, against
@@ -53,6 +67,7 @@ def build_fortran(verbose: bool = False) -> str | None:
 
 def build(verbose: bool = False) -> None:
     build_lib(verbose)
+    build_stream_probe(verbose)
     build_fortran(verbose)
 
 
