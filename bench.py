@@ -161,7 +161,7 @@ def solve_bytes(s, args, mode, cycles):
     n1 = N[0]
     cyc += n1 * (6 * 8 + (4 if mixed else 0))                      # FP64 residual + norms
     if mixed:
-        cyc += n1 * 20                                             # x64 += e32
+        cyc += n1 * 12                                             # x + e32 read, x + e written
     setup = n1 * (32 + 6 * 8 + (4 if mixed else 0) + 16 + 56)
     return cycles * cyc + setup
 

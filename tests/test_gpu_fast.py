@@ -279,6 +279,33 @@ def test_fp64_line_kernels_converge(kernel, cfg, scale, monkeypatch):
     assert err < 1e-9, err
 
 
+@pytest.mark.parametrize("cfg,scale", [("c3", 8), ("c4", 4)])
+def test_mixed_iterate_parity(cfg, scale, monkeypatch):
+    """Mixed precision keeps two level-1 iterate arrays (each cycle's residual pass writes x + e
+    into the other one): solves of 1, 2 and 3 cycles from p = 0 give bitwise the first rows of a
+    4-cycle solve's history, and each returned residual is b - A f of its returned pressure (an
+    independent sparse product, to the rounding of two FP64 evaluation orders)."""
+    import torch
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg")
+    p, o, s, coeffs, b = setup(cfg, scale)
+    A, idx = assemble_level1(o, coeffs)
+    m = interior_mask(o, 0)
+    bh = b.cpu().numpy()[: o.n1][m]
+    kw = dict(tolf=0.0, rtol=0.0, smoother=1, precision=1, numpre=1, numpost=1, omega=1.35,
+              omega_c=1.0, sweeps_per_iter=3)
+    runs = {}
+    for n in (4, 1, 2, 3):
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b.clone(), res, max_cycles=n, **kw)
+        runs[n] = (hist, f.cpu().numpy()[: o.n1][m], res.cpu().numpy()[: o.n1][m])
+    for n in (1, 2, 3):
+        hist, x, r = runs[n]
+        assert np.array_equal(hist, runs[4][0][:n])
+        scale = np.abs(bh) + abs(A) @ np.abs(x)
+        assert np.all(np.abs(r - (bh - A @ x)) <= 1e-13 * scale)
+
+
 @pytest.mark.parametrize("cfg,scale", [("c1", 2), ("c3", 4)])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_per_block_launches_agree(cfg, scale, prec, monkeypatch):
