@@ -235,6 +235,7 @@ def step_parts(p, s, args, reps=5):
     out["source"] = clock(lambda: s.source(fl["u"], fl["v"], fl["w"], fl["rho"], fl["rhonp1"],
                                            p.dtime, b))
     out["solve_same_operator"] = clock(lambda: solve(s, b, args, "mixed", args.rtol))
+    f = solve(s, b, args, "mixed", args.rtol)[0]
 
     def new_op_solve():
         s.update_density(fl["rhonp1"])
@@ -245,8 +246,24 @@ def step_parts(p, s, args, reps=5):
                                                              p.dtime))
     with tempfile.TemporaryDirectory() as td:
         out["file_3 binary REAL*4"] = clock(lambda: s.file_3(os.path.join(td, "p"), f), n=2)
+    # the differentiable solve's backward (fortranutils_amd/adjoint.py): the transposed-operator
+    # FP64 multigrid to a 1e-13 reduction and the density gradient, on the same operator
+    from fortranutils_amd.adjoint import ADJ_SOLVE
+    g = torch.zeros_like(b)
+    g[:n1] = b[:n1]
+    lam = torch.zeros_like(b)
+    res = torch.zeros_like(b)
+    cyc = []
+
+    def adjoint():
+        lam.zero_()
+        cyc.append(len(s.solve_adjoint(lam, g, res, **ADJ_SOLVE)))
+        s.rho_adjoint(lam[:n1].contiguous(), f[:n1].contiguous())
+    out["adjoint solve + density gradient"] = clock(adjoint, n=2)
+    out["adjoint cycles"] = float(cyc[-1])
     for k, v in out.items():
-        print(f"# step part {k}: {v:.2f} ms", file=sys.stderr, flush=True)
+        print(f"# step part {k}: {v:.2f}{'' if 'cycles' in k else ' ms'}", file=sys.stderr,
+              flush=True)
     del fl
     return out
 
