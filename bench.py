@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--numpre", type=int, default=1)
     ap.add_argument("--numpost", type=int, default=1)
     ap.add_argument("--spi", type=int, default=3, help="line-relaxation sweeps per smoothing iteration")
+    ap.add_argument("--fine-post", type=int, default=2,
+                    help="post-smoothing sweeps on level 1 in a lean cycle (0 = numpost * spi)")
     ap.add_argument("--numcits", type=int, default=20)
     ap.add_argument("--omega", type=float, default=1.35)
     ap.add_argument("--omega-c", type=float, default=1.0)
@@ -121,7 +123,7 @@ def stream_ceiling(reps=5):
     return out
 
 
-def solve_bytes(s, args, mode, cycles):
+def solve_bytes(s, args, mode, cycles, post0=None):
     """Algorithmic HBM bytes of one solve of ``cycles`` cycles with this schedule, per kernel and
     level (halo swaps, O(surface), count 0; symmetric operator, as the solver checks):
       half-sweep: 10 words per updated cell (9 with omega = 1); on a level whose FP32 segmented
@@ -136,7 +138,9 @@ def solve_bytes(s, args, mode, cycles):
       level 1, mixed: FP64 residual + norms (6 FP64 words) writing the FP32 right-hand side, and
         x64 += e32 (FP64 x read and written, e read); FP64: the residual without the rhs;
       per solve: f and b to the split layout, the first residual, the solution back and its
-        natural-layout residual (ap, ae, an, ar, b, x read, r written)."""
+        natural-layout residual (ap, ae, an, ar, b, x read, r written).
+    post0: level 1's post-smoothing sweeps per cycle, on average over the solve's lean and full
+    cycles (default numpost * spi)."""
     L = s.nlev
     N = [float(np.prod(np.asarray(s.nijkp1[l]) - 2, axis=1).sum()) for l in range(L)]
     mixed = mode == "mixed"
@@ -148,7 +152,8 @@ def solve_bytes(s, args, mode, cycles):
     cyc = 0.0
     for l in range(L - 1):
         hb = hw * w[l] - (6 * w[l] - 12 if c16[l] and int(tw[l]) in (8, 16) else 0)
-        cyc += (args.numpre + args.numpost) * args.spi * N[l] * hb
+        npost = post0 if (l == 0 and post0 is not None) else args.numpost * args.spi
+        cyc += (npre + npost) * N[l] * hb
         if mixed and npre >= 1 and int(tw[l]) in (8, 16):
             cyc -= 0.5 * N[l] * ((hw - 8) + (hw - 9)) * w[l]     # zero-start half-sweeps
         elif mixed or l > 0:
@@ -195,7 +200,8 @@ def solve(s, b, args, mode, rtol, profile=False):
         kw.update(smoother=0, precision=0, numpre=2, numpost=2, omega=1.6, omega_c=1.6)
     else:
         kw.update(smoother=1, precision=1 if mode == "mixed" else 0, numpre=args.numpre,
-                  numpost=args.numpost, sweeps_per_iter=args.spi, omega=args.omega,
+                  numpost=args.numpost, sweeps_per_iter=args.spi, fine_post=args.fine_post,
+                  omega=args.omega,
                   omega_c=args.omega_c)
     if profile:
         s.profile(True)
@@ -597,8 +603,14 @@ def main():
     # old x is read).  With binary16 coefficients (mixed precision) the 6 operator words are 2
     # bytes each (ap replaced by the diagonal excess d): 28 bytes over-relaxed, 24 without.
     roof = None
+    post0 = None
     if args.mode != "exact":
-        _, _, (ms, nl, cells) = solve(s, b, args, args.mode, args.rtol, profile=True)
+        _, hist_p, (ms, nl, cells) = solve(s, b, args, args.mode, args.rtol, profile=True)
+        # level 1's plain half-sweeps: per cycle 2 numpre spi (less the 2 zero-start ones of the
+        # segmented kernel) and 2 per post-smoothing sweep, lean or full
+        nzs = 2 if args.mode == "mixed" and int(s.tile_widths()[1][0]) in (8, 16) else 0
+        ncp = max(len(hist_p), 1)
+        post0 = max(0.0, (nl - ncp * (2 * args.numpre * args.spi - nzs)) / (2.0 * ncp))
         word = 4 if args.mode == "mixed" else 8
         words = 9 if args.omega == 1.0 else 10
         tm = int(s.tile_widths()[1 if word == 4 else 0][0])
@@ -671,7 +683,8 @@ def main():
                   file=sys.stderr, flush=True)
 
     # the whole solve against the HBM roofline: algorithmic bytes of the schedule / time
-    sb = solve_bytes(s, args, args.mode, cycles / args.steps) if args.mode != "exact" else None
+    sb = (solve_bytes(s, args, args.mode, cycles / args.steps, post0) if args.mode != "exact"
+          else None)
     solve_roof = None
     if sb:
         ach = sb / (dt / args.steps)
@@ -696,7 +709,8 @@ def main():
                                f"{s.nlev} levels, solve to {args.rtol:g} residual reduction",
                    "mode": args.mode, "smoother": "zebra y-line relaxation" if args.mode != "exact" else
                    "reference lexicographic SOR", "numpre": args.numpre, "numpost": args.numpost,
-                   "sweeps_per_iter": args.spi, "omega": args.omega, "numcits": args.numcits},
+                   "sweeps_per_iter": args.spi, "fine_post": args.fine_post, "omega": args.omega,
+                   "numcits": args.numcits},
         "time_to_rtol_ms": ms_step,
         "cycles_per_solve": cycles / args.steps,
         "roofline": roof,

@@ -423,6 +423,7 @@ extern "C" void lmgx_default_params(lmgx_params *p)
     p->sweeps_per_iter = 3;
     p->coarse_op = 1;
     p->coarse_solve = 1;
+    p->fine_post = 0;
 }
 
 static void free_fast(lmgx_handle *h);
@@ -875,7 +876,8 @@ static int check_params(const lmgx_handle *h, const lmgx_params *p)
 {
     if (!p) return fail(LMGX_E_ARG, "NULL params");
     if (p->cycle != 1 && p->cycle != 2) return fail(LMGX_E_ARG, "cycle must be 1 (V) or 2 (F)");
-    if (p->numpre < 0 || p->numpost < 0 || p->numcits < 0 || p->max_cycles < 0)
+    if (p->numpre < 0 || p->numpost < 0 || p->numcits < 0 || p->max_cycles < 0 ||
+        p->fine_post < 0)
         return fail(LMGX_E_ARG, "negative iteration count");
     if (p->smoother != LMGX_SMOOTH_EXACT && p->smoother != LMGX_SMOOTH_LINE &&
         p->smoother != LMGX_SMOOTH_SPORLX2)

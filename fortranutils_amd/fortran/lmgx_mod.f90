@@ -29,6 +29,7 @@ module lmgx_mod
     integer(c_int) :: cycle, numpre, numpost, numcits, max_cycles
     real(c_double) :: omega, omega_c, tolf, rtol
     integer(c_int) :: smoother, precision, check_every, sweeps_per_iter, coarse_op, coarse_solve
+    integer(c_int) :: fine_post
   end type
 
   interface

@@ -36,7 +36,8 @@ class Params(C.Structure):
                 ("numcits", C.c_int), ("max_cycles", C.c_int), ("omega", C.c_double),
                 ("omega_c", C.c_double), ("tolf", C.c_double), ("rtol", C.c_double),
                 ("smoother", C.c_int), ("precision", C.c_int), ("check_every", C.c_int),
-                ("sweeps_per_iter", C.c_int), ("coarse_op", C.c_int), ("coarse_solve", C.c_int)]
+                ("sweeps_per_iter", C.c_int), ("coarse_op", C.c_int), ("coarse_solve", C.c_int),
+                ("fine_post", C.c_int)]
 
 
 class _Topo(C.Structure):

@@ -76,6 +76,12 @@ typedef struct lmgx_params {
                              1 = built from level 1 (half the piecewise-constant Galerkin product) */
     int coarse_solve;     /* fast path: 0 = numcits relaxations on the coarsest level (solversc),
                              1 = exact solve (Cholesky inverse) when the coarsest operator is SPD */
+    int fine_post;        /* fast path: post-smoothing line sweeps on level 1 in a "lean" cycle,
+                             0 = numpost * sweeps_per_iter (every cycle alike).  With fewer sweeps
+                             than that, a cycle is lean unless the stop test (rtol) is predicted
+                             to end the solve with that cycle only when it is done in full: the
+                             post-smoothing of level 1 mostly cleans the residual the stop test
+                             measures, which the next cycle's pre-smoothing cleans anyway. */
 } lmgx_params;
 
 /* Default parameters: the reference's hard-wired knobs (code/defaults.F:59-71). */

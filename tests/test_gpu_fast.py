@@ -279,6 +279,38 @@ def test_fp64_line_kernels_converge(kernel, cfg, scale, monkeypatch):
     assert err < 1e-9, err
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("cfg,scale", [("c4", 4), ("c1", 4), ("c0", 2)])
+def test_lean_cycles(cfg, scale, prec, monkeypatch):
+    """Level-1 post-smoothing with 2 of its 3 sweeps in cycles the residual test is not predicted
+    to end (fine_post = 2): the same pressure as the full cycles within 1e-9 of scipy's direct
+    solve, at most one more cycle; fine_post = 3 (= numpost * sweeps_per_iter) is the full
+    schedule bit for bit."""
+    import torch
+    import scipy.sparse.linalg as spla
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg" if prec else "64")
+    p, o, s, coeffs, b = setup(cfg, scale)
+    A, idx = assemble_level1(o, coeffs)
+    bh = b.cpu().numpy()[: o.n1]
+    m = interior_mask(o, 0)
+    x_ref = spla.spsolve(A.tocsc(), bh[m])
+    out = {}
+    for fp in (0, 3, 2):
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b, res, max_cycles=40, tolf=0.0, rtol=1e-11, smoother=1,
+                           precision=prec, numpre=1, numpost=1, sweeps_per_iter=3, fine_post=fp,
+                           omega=1.35, omega_c=1.0)
+        x = f.cpu().numpy()[: o.n1][m]
+        out[fp] = (len(hist), x, hist)
+        print(cfg, prec, "fine_post", fp, "cycles", len(hist))
+    assert np.array_equal(out[3][1], out[0][1]) and np.array_equal(out[3][2], out[0][2])
+    assert out[2][0] <= out[0][0] + 1
+    for fp in (0, 2):
+        err = np.linalg.norm(out[fp][1] - x_ref) / np.linalg.norm(x_ref)
+        assert err < 1e-9, (fp, err)
+
+
 @pytest.mark.parametrize("cfg,scale", [("c3", 8), ("c4", 4)])
 def test_mixed_iterate_parity(cfg, scale, monkeypatch):
     """Mixed precision keeps two level-1 iterate arrays (each cycle's residual pass writes x + e
