@@ -60,6 +60,7 @@ struct LevelDev {
     std::vector<long> bt_sg, bt_sg8, bt_z64, bt_z128, bt_rr, bt_rp;
     long tiles_int = 0, tiles_all = 0, tiles_rb = 0, tiles_ln = 0, nlines = 0;
     long tiles_z64 = 0, tiles_z128 = 0, tiles_rr = 0, tiles_rp = 0, tiles_sg = 0, tiles_sg8 = 0;
+    long n2s = 0;                       // slots per parity of the parity-split 2-D tables
     int njmax = 0;                      // longest j-line (interior rows) of the level
     int smax = 0;                       // largest hyperplane index i+j+k (1-based)
     std::vector<Face> h_fc;
@@ -110,6 +111,11 @@ struct lmgx_handle {
     // geometry (lmgx_build_operator)
     double *d_met = nullptr;            // [7][n2d]: diu djv rdiv rdju didjp rdidjp rp
     double *d_dkp = nullptr, *d_dkw = nullptr;
+    // level 1's metric form (see MetF): packed parity-split face factors [2 n2s] times mf_gsc
+    // (0: not available), dkp(k) and 1 / dkw(k) [2][n1d]; written with the metrics
+    uint2 *d_mft = nullptr;
+    float *d_mfk = nullptr;
+    double mf_gsc = 0.0;
     FastBufs *fast = nullptr;
     // profiling of the smoother
     bool prof = false;
@@ -221,6 +227,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
         LevelDev &D = h->lev[l];
         D.h_lb.resize(h->nblk);
         long ti = 0, ta = 0, tr = 0, tl = 0, tl2 = 0, tl4 = 0, trr = 0, trp = 0, tsg = 0, tsg8 = 0;
+        long t2s = 0;
         long sl = 0, ns = 0;
         double np = 0.0;
         long npi = 0;
@@ -280,6 +287,8 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
             B.o1 = h->off1d[l][b];
             B.oxy = h->offxy[l][b];
             B.oz = h->offz[l][b];
+            B.o2s = t2s;
+            t2s += (long)n[1] * B.hp;
             np = np + (double)((long)(n[0] - 2) * (n[1] - 2) * (n[2] - 2));
             npi += (long)(n[0] - 2) * (n[1] - 2) * (n[2] - 2);
             smax = std::max(smax, n[0] - 1 + n[1] - 1 + n[2] - 1);
@@ -295,6 +304,7 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
         D.tiles_rp = trp;
         D.tiles_sg = tsg;
         D.tiles_sg8 = tsg8;
+        D.n2s = t2s;
         D.nlines = sl;
         D.smax = smax;
         D.npts_int = npi;
@@ -442,6 +452,7 @@ extern "C" int lmgx_free(lmgx_handle *h)
     hipFree(h->d_swap); hipFree(h->d_partial); hipFree(h->d_partial2); hipFree(h->d_scalar);
     hipFree(h->d_hist);
     hipFree(h->d_met); hipFree(h->d_dkp); hipFree(h->d_dkw);
+    hipFree(h->d_mft); hipFree(h->d_mfk);
     if (h->h_pinned) hipHostFree(h->h_pinned);
     free_fast(h);
     for (auto &e : h->prof_ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }

@@ -14,6 +14,10 @@
 #define LMGX_SMAX 16    // segmented line solve: rows per segment held in registers, 16 segments
                         // at most, so lines up to 256 rows
 
+#ifndef LMGX_MFLA
+#define LMGX_MFLA 3     // metric-form line relaxation: rows of loads issued ahead
+#endif
+
 #define BCT_PERIODIC 12
 #define BCT_INTERFACE 13
 
@@ -45,6 +49,7 @@ struct LevBlk {
     long o1;         // offset of the block's 1-D k metric arrays
     long oxy;        // offset of the block's vertex array (nip1+1)*(njp1+1)
     long oz;         // offset of the block's k-vertex array (nkp1+1)
+    long o2s;        // offset of the block's parity-split 2-D tables (njp1 rows of hp, per parity)
     // Halo stores made by the segmented line solve itself.  Side q = 0..3 is the interior plane
     // i = 1, i = nip1-2, k = 1, k = nkp1-2; when it is the source of a partner's face-interior
     // halo, fz_off[q] >= 0 is that block's off_s and the halo point of source point (I,J,K) is

@@ -443,6 +443,13 @@ class Solver:
         _check(lib().lmgx_coef16_levels(self.h, _ip(out)))
         return out[: self.nlev]
 
+    def metric_form(self) -> bool:
+        """True when the last mixed solve took level 1's builder operator in its metric form
+        (couplings recomputed from binary16 densities and face factors)."""
+        out = C.c_int(0)
+        _check(lib().lmgx_metric_form(self.h, C.byref(out)))
+        return bool(out.value)
+
     # ---- profiling ----------------------------------------------------------------------
     def profile(self, enable=True):
         _check(lib().lmgx_profile(self.h, int(enable)))

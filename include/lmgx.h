@@ -289,6 +289,18 @@ int lmgx_fused_halos(const lmgx_handle *h, int *out);
  * is set. */
 int lmgx_coef16_levels(const lmgx_handle *h, int *out);
 
+/* Whether level 1's FP32 segmented line relaxation and restriction residual (mixed precision)
+ * take the builder's operator in its metric form: *out = 1 when the last mixed solve recomputed
+ * each cell's couplings from binary16 copies of the densities (times a power of two) and the
+ * grid's face factors (binary16, one 64-bit word per cell), g dk / ((rho_a + rho_b) / 2) as
+ * lmgx_build_operator builds them, instead of reading stored couplings.  Asked for with
+ * LMGX_METRIC=1 in the environment at the first mixed solve after an operator change (the stored
+ * binary16 couplings are the default), and only for the operator of lmgx_build_operator /
+ * lmgx_update_density_device, when the densities are positive and their range and the factors'
+ * fit binary16 (2^28).  No counterpart in the reference (an extension of its
+ * coeff_p_block.F:104-137 coefficients with the 1/rho_face scaling). */
+int lmgx_metric_form(const lmgx_handle *h, int *out);
+
 /* The fast path's coarsest-level solve: out[0] = 1 when it is the exact solve (the coarsest
  * operator factored, its inverse applied per cycle), out[1] = 1 when the factor and inverse came
  * from the banded Cholesky (components in reverse Cuthill-McKee order, half-bandwidth out[2] at
