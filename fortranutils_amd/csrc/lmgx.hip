@@ -116,6 +116,8 @@ struct lmgx_handle {
     uint2 *d_mft = nullptr;
     float *d_mfk = nullptr;
     double mf_gsc = 0.0;
+    // the same factors in FP64 (see MetD), exact quotients as the builder forms them [2 n2s]
+    double4 *d_mfd = nullptr;
     FastBufs *fast = nullptr;
     // profiling of the smoother
     bool prof = false;
@@ -452,7 +454,7 @@ extern "C" int lmgx_free(lmgx_handle *h)
     hipFree(h->d_swap); hipFree(h->d_partial); hipFree(h->d_partial2); hipFree(h->d_scalar);
     hipFree(h->d_hist);
     hipFree(h->d_met); hipFree(h->d_dkp); hipFree(h->d_dkw);
-    hipFree(h->d_mft); hipFree(h->d_mfk);
+    hipFree(h->d_mft); hipFree(h->d_mfk); hipFree(h->d_mfd);
     if (h->h_pinned) hipHostFree(h->h_pinned);
     free_fast(h);
     for (auto &e : h->prof_ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }

@@ -443,12 +443,12 @@ class Solver:
         _check(lib().lmgx_coef16_levels(self.h, _ip(out)))
         return out[: self.nlev]
 
-    def metric_form(self) -> bool:
-        """True when the last mixed solve took level 1's builder operator in its metric form
-        (couplings recomputed from binary16 densities and face factors)."""
-        out = C.c_int(0)
-        _check(lib().lmgx_metric_form(self.h, C.byref(out)))
-        return bool(out.value)
+    def metric_form(self):
+        """(FP32 relaxation, FP64 residual): whether the last solve took level 1's builder
+        operator in its metric form (couplings recomputed from the densities and face factors)."""
+        out = np.zeros(2, np.int32)
+        _check(lib().lmgx_metric_form(self.h, _ip(out)))
+        return bool(out[0]), bool(out[1])
 
     # ---- profiling ----------------------------------------------------------------------
     def profile(self, enable=True):

@@ -289,8 +289,14 @@ int lmgx_fused_halos(const lmgx_handle *h, int *out);
  * is set. */
 int lmgx_coef16_levels(const lmgx_handle *h, int *out);
 
-/* Whether level 1's FP32 segmented line relaxation and restriction residual (mixed precision)
- * take the builder's operator in its metric form: *out = 1 when the last mixed solve recomputed
+/* Whether level 1 takes the builder's operator in its metric form, out[2]:
+ * out[1] = 1 when the last fast solve's level-1 FP64 residual passes formed the couplings of the
+ * cells off the blocks' boundary planes from the densities (FP64) and the builder's face factors,
+ * with its operations in its order (bitwise its operator), instead of reading them: for an
+ * operator of lmgx_build_operator / lmgx_update_density_device built with densities, when
+ * LMGX_METRIC64=1 in the environment at the first solve after an operator change (off by default).
+ * out[0] = 1 when level 1's FP32 segmented line relaxation and restriction residual (mixed
+ * precision) took it in metric form as well: when the last mixed solve recomputed
  * each cell's couplings from binary16 copies of the densities (times a power of two) and the
  * grid's face factors (binary16, one 64-bit word per cell), g dk / ((rho_a + rho_b) / 2) as
  * lmgx_build_operator builds them, instead of reading stored couplings.  Asked for with
