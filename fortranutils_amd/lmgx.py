@@ -443,6 +443,13 @@ class Solver:
         _check(lib().lmgx_coef16_levels(self.h, _ip(out)))
         return out[: self.nlev]
 
+    def finished_in_cycle(self) -> bool:
+        """True when the last fast solve ended on a cycle that wrote the returned solution and
+        residual itself (no final conversion and residual pass)."""
+        out = C.c_int(0)
+        _check(lib().lmgx_finished_in_cycle(self.h, C.byref(out)))
+        return bool(out.value)
+
     def metric_form(self):
         """(FP32 relaxation, FP64 residual): whether the last solve took level 1's builder
         operator in its metric form (couplings recomputed from the densities and face factors)."""

@@ -289,6 +289,15 @@ int lmgx_fused_halos(const lmgx_handle *h, int *out);
  * is set. */
 int lmgx_coef16_levels(const lmgx_handle *h, int *out);
 
+/* *out = 1 when the last fast solve ended on a cycle that wrote the returned solution and
+ * residual itself: with lean cycles (lmgx_params.fine_post), the cycles the prediction runs in
+ * full are those expected to end the solve, and their level-1 residual pass also stores the
+ * residual and the solution into the caller's natural-layout arrays, so that a solve ending on
+ * one skips the final conversion and residual pass (the same values, bit for bit).
+ * LMGX_NO_FIN=1 in the environment at a solve keeps the final passes.  No counterpart in the
+ * reference. */
+int lmgx_finished_in_cycle(const lmgx_handle *h, int *out);
+
 /* Whether level 1 takes the builder's operator in its metric form, out[2]:
  * out[1] = 1 when the last fast solve's level-1 FP64 residual passes formed the couplings of the
  * cells off the blocks' boundary planes from the densities (FP64) and the builder's face factors,

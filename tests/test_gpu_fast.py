@@ -311,6 +311,40 @@ def test_lean_cycles(cfg, scale, prec, monkeypatch):
         assert err < 1e-9, (fp, err)
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c4", 4), ("c0", 1)])
+def test_finishing_cycle_writes_the_returned_arrays(cfg, scale, prec, monkeypatch):
+    """With lean cycles, a cycle predicted to end the solve runs in full and its level-1 pass also
+    writes the returned solution and residual; a solve that ends on it skips the final conversion
+    and residual pass.  The returned solution (halos included, from a nonzero initial guess),
+    residual and history are bitwise those of the same solve with the final passes
+    (LMGX_NO_FIN=1).  Over a few tolerances (whether the last cycle is a full one depends on the
+    prediction), at least one solve must end on such a cycle."""
+    import torch
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg" if prec else "64")
+    p, o, s, coeffs, b = setup(cfg, scale)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    f0 = torch.randn(o.ntot, dtype=torch.float64, device="cuda", generator=g)
+    fins = []
+    for rtol in (1e-9, 1e-10, 1e-11, 1e-12):
+        out = []
+        for mode in ("1", "0"):
+            monkeypatch.setenv("LMGX_NO_FIN", mode)
+            f = f0.clone()
+            res = torch.zeros_like(f)
+            hist = s.lmg_solve(f, b, res, max_cycles=40, tolf=0.0, rtol=rtol, smoother=1,
+                               precision=prec, numpre=1, numpost=1, sweeps_per_iter=3,
+                               fine_post=2, omega=1.35, omega_c=1.0)
+            fin = s.finished_in_cycle()
+            print(cfg, prec, rtol, "no_fin", mode, "cycles", len(hist), "finished in cycle", fin)
+            assert not (fin and mode == "1")
+            out.append((f.cpu().numpy(), res.cpu().numpy(), hist))
+        fins.append(fin)
+        for x, y in zip(*out):
+            assert np.array_equal(x, y)
+    assert any(fins)
+
+
 @pytest.mark.parametrize("cfg,scale", [("c3", 8), ("c4", 4)])
 def test_mixed_iterate_parity(cfg, scale, monkeypatch):
     """Mixed precision keeps two level-1 iterate arrays (each cycle's residual pass writes x + e
