@@ -345,6 +345,32 @@ def test_finishing_cycle_writes_the_returned_arrays(cfg, scale, prec, monkeypatc
     assert any(fins)
 
 
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c4", 4), ("c0", 1)])
+def test_zero_initial_iterate(cfg, scale, prec, monkeypatch):
+    """A zero initial iterate (interior; nonzero halos included) is found while f is converted,
+    and the first residual pass then reads only b and ap: the same solve bit for bit as with the
+    full first pass (LMGX_NO_ZERO_START=1) - solution, residual, history."""
+    import torch
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg" if prec else "64")
+    p, o, s, coeffs, b = setup(cfg, scale)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    f0 = torch.randn(o.ntot, dtype=torch.float64, device="cuda", generator=g)
+    m = torch.from_numpy(interior_mask(o, 0)).cuda()
+    f0[: o.n1][m] = 0.0
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("LMGX_NO_ZERO_START", mode)
+        f = f0.clone()
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b, res, max_cycles=40, tolf=0.0, rtol=1e-10, smoother=1,
+                           precision=prec, numpre=1, numpost=1, sweeps_per_iter=3, fine_post=2,
+                           omega=1.35, omega_c=1.0)
+        out.append((f.cpu().numpy(), res.cpu().numpy(), hist))
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("cfg,scale", [("c3", 8), ("c4", 4)])
 def test_mixed_iterate_parity(cfg, scale, monkeypatch):
     """Mixed precision keeps two level-1 iterate arrays (each cycle's residual pass writes x + e
