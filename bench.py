@@ -628,7 +628,8 @@ def main():
         fp = "float" if word == 4 else "double"
         fz = "true" if s.fused_halos()[1 if word == 4 else 0][0] else "false"
         segk = {16: "k_zebra_seg", 8: "k_zebra_seg8"}.get(tm)
-        key = (f"{segk}<{fp}, 1, {rl}, 0, {'true' if h16 else 'false'}>" if segk
+        # (the segmented kernels' last argument: level 1 in metric form, off by default)
+        key = (f"{segk}<{fp}, 1, {rl}, 0, {'true' if h16 else 'false'}, false>" if segk
                else f"k_zebra_line<{fp}, 1, {tm}, {rl}, {fz}>")
         tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
