@@ -14,6 +14,9 @@
 #define LMGX_SMAX 16    // segmented line solve: rows per segment held in registers, 16 segments
                         // at most, so lines up to 256 rows
 
+#ifndef LMGX_SEGLA
+#define LMGX_SEGLA 6    // segmented line relaxation on binary16 couplings: rows of loads issued ahead
+#endif
 #ifndef LMGX_MFLA
 #define LMGX_MFLA 3     // metric-form line relaxation: rows of loads issued ahead
 #endif
