@@ -371,6 +371,39 @@ def test_zero_initial_iterate(cfg, scale, prec, monkeypatch):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c1", 2), ("c0", 1), ("c4", 4), ("c2", 8)])
+def test_mixed_reads_natural_b(cfg, scale, monkeypatch):
+    """In mixed precision the level-1 FP64 passes read b from the caller's natural-layout array
+    and b is not converted to the split layout: bitwise the solve that converts it and reads the
+    copy (LMGX_SPLIT_B=1) - solution, residual, history - from a zero and from a random initial
+    iterate, for solves that end on a finishing cycle or not (several tolerances) and for a
+    fixed number of full cycles (rtol 0: no lean cycles, the final residual pass)."""
+    import torch
+    monkeypatch.setenv("LMGX_LINE_KERNEL", "seg")
+    p, o, s, coeffs, b = setup(cfg, scale)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    f_rand = torch.randn(o.ntot, dtype=torch.float64, device="cuda", generator=g)
+    fins = []
+    for f0 in (torch.zeros_like(f_rand), f_rand):
+        for rtol, cyc in ((1e-10, 40), (1e-11, 40), (1e-12, 40), (0.0, 3)):
+            out = []
+            for mode in ("1", "0"):
+                monkeypatch.setenv("LMGX_SPLIT_B", mode)
+                f = f0.clone()
+                res = torch.zeros_like(f)
+                hist = s.lmg_solve(f, b, res, max_cycles=cyc, tolf=0.0, rtol=rtol, smoother=1,
+                                   precision=1, numpre=1, numpost=1, sweeps_per_iter=3,
+                                   fine_post=2, omega=1.35, omega_c=1.0)
+                out.append((f.cpu().numpy(), res.cpu().numpy(), hist))
+                if mode == "0":
+                    fins.append(s.finished_in_cycle())
+            print(cfg, "rtol", rtol, "cycles", len(out[0][2]), len(out[1][2]),
+                  "finished in cycle", fins[-1])
+            for x, y in zip(*out):
+                assert np.array_equal(x, y)
+    assert any(fins) and not all(fins)
+
+
 @pytest.mark.parametrize("cfg,scale", [("c3", 8), ("c4", 4)])
 def test_mixed_iterate_parity(cfg, scale, monkeypatch):
     """Mixed precision keeps two level-1 iterate arrays (each cycle's residual pass writes x + e
