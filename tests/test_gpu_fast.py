@@ -404,6 +404,30 @@ def test_mixed_reads_natural_b(cfg, scale, monkeypatch):
     assert any(fins) and not all(fins)
 
 
+@pytest.mark.parametrize("kernel", ["64", "seg8"])
+@pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c1", 2), ("c0", 1), ("c4", 4)])
+def test_fp64_restriction_formed_diagonal(cfg, scale, kernel, monkeypatch):
+    """The FP64 solve's level-1 restriction residual on the builder's operator forms each
+    interior diagonal from the couplings instead of reading it (8 bytes fewer per cell): bitwise
+    the solve that reads it (LMGX_RESTRICT_AP=1) - solution, residual, history."""
+    import torch
+    monkeypatch.setenv("LMGX_LINE_KERNEL", kernel)
+    out = []
+    for mode in ("1", "0"):
+        monkeypatch.setenv("LMGX_RESTRICT_AP", mode)
+        p, o, s, coeffs, b = setup(cfg, scale)
+        f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+        res = torch.zeros_like(f)
+        hist = s.lmg_solve(f, b, res, max_cycles=40, tolf=0.0, rtol=1e-11, smoother=1,
+                           precision=0, numpre=1, numpost=1, sweeps_per_iter=3, fine_post=2,
+                           omega=1.35, omega_c=1.0)
+        print(cfg, kernel, "restrict ap", mode, "cycles", len(hist), s.tile_widths().tolist())
+        assert len(hist) < 40
+        out.append((f.cpu().numpy(), res.cpu().numpy(), hist))
+    for x, y in zip(*out):
+        assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("cfg,scale", [("c3", 8), ("c4", 4)])
 def test_mixed_iterate_parity(cfg, scale, monkeypatch):
     """Mixed precision keeps two level-1 iterate arrays (each cycle's residual pass writes x + e
