@@ -910,6 +910,13 @@ extern "C" int lmgx_solve_device(lmgx_handle *h, double *d_f, double *d_b, doubl
                                  const lmgx_params *prm, double *history, int *ncycles_out)
 {
     if (!h || !d_f || !d_b || !d_res) return fail(LMGX_E_ARG, "NULL argument");
+    // the solve reads b and writes f and residu throughout (the fast path reads b where the caller
+    // holds it and may write f and residu in a cycle that turns out not to be the last)
+    auto overlap = [&](const double *x, const double *y) {
+        return x < y + h->ntot && y < x + h->ntot;
+    };
+    if (overlap(d_f, d_b) || overlap(d_f, d_res) || overlap(d_b, d_res))
+        return fail(LMGX_E_ARG, "f, b and residu must not overlap");
     CHK(check_params(h, prm));
     int nc = 0;
     CHK(ensure_hist(h, (long)std::max(1, prm->max_cycles) * h->nlev * 4 + 64));

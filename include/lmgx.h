@@ -119,7 +119,8 @@ int lmgx_set_operator_device(lmgx_handle *h, const double *const d_coeff[7]);
 int lmgx_solve(lmgx_handle *h, double *f, double *b, double *residu, const lmgx_params *prm,
                double *history, int *ncycles_out);
 /* Same on caller-owned device arrays (ntot doubles each, stream-ordered on the handle's stream,
- * synchronised before return). */
+ * synchronised before return).  The three arrays must not overlap (LMGX_E_ARG): the solve reads b
+ * in place while it writes f and residu. */
 int lmgx_solve_device(lmgx_handle *h, double *d_f, double *d_b, double *d_res,
                       const lmgx_params *prm, double *history, int *ncycles_out);
 

@@ -794,3 +794,23 @@ def test_tuning_table_reproduces_a_solve(prec):
     f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
     with pytest.raises(lmgx.LmgxError, match="cannot run line kernel 16"):
         s.lmg_solve(f, b, torch.zeros_like(f), **kw)
+
+
+def test_overlapping_solve_arrays_are_rejected():
+    """The solve reads b where the caller holds it while it writes f and the residual, so the
+    device entry point refuses overlapping arrays (LMGX_E_ARG) before touching any of them."""
+    import torch
+    from fortranutils_amd import lmgx
+    p, o, s, coeffs, b = setup("c4", 4)
+    f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+    big = torch.zeros(2 * o.ntot, dtype=torch.float64, device="cuda")
+    b0 = b.clone()
+    for ff, bb, rr in ((f, b, b), (b, b, f), (f, big[: o.ntot], big[o.ntot // 2: o.ntot // 2 + o.ntot])):
+        with pytest.raises(lmgx.LmgxError, match="must not overlap"):
+            s.lmg_solve(ff, bb, rr, max_cycles=3, tolf=0.0, rtol=1e-10, smoother=1, precision=1,
+                        numpre=1, numpost=1, omega=1.35, omega_c=1.0, sweeps_per_iter=3)
+    assert torch.equal(b, b0)
+    res = torch.zeros_like(f)
+    hist = s.lmg_solve(f, b, res, max_cycles=40, tolf=0.0, rtol=1e-10, smoother=1, precision=1,
+                       numpre=1, numpost=1, omega=1.35, omega_c=1.0, sweeps_per_iter=3)
+    assert len(hist) < 40
