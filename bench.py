@@ -628,9 +628,13 @@ def main():
         fp = "float" if word == 4 else "double"
         fz = "true" if s.fused_halos()[1 if word == 4 else 0][0] else "false"
         segk = {16: "k_zebra_seg", 8: "k_zebra_seg8"}.get(tm)
+        # the finest level's instances: 1 where a half-sweep moves more than the 256 MiB
+        # Infinity Cache (streamed past the caches), else 2 (the library's rule, LMGX_STREAM_MIB)
+        lim = float(os.environ.get("LMGX_STREAM_MIB", "256")) * 1048576.0
+        fine = 1 if 0.5 * cells_launch * 2 * (28 if word == 4 else 104) > lim else 2
         # (the segmented kernels' last argument: level 1 in metric form, off by default)
-        key = (f"{segk}<{fp}, 1, {rl}, 0, {'true' if h16 else 'false'}, false>" if segk
-               else f"k_zebra_line<{fp}, 1, {tm}, {rl}, {fz}>")
+        key = (f"{segk}<{fp}, {fine}, {rl}, 0, {'true' if h16 else 'false'}, false>" if segk
+               else f"k_zebra_line<{fp}, {fine}, {tm}, {rl}, {fz}>")
         tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=achieved / PEAK_HBM, traffic=tr,
