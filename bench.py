@@ -95,31 +95,35 @@ def copy_calib():
 
 
 # read/write mixes of the streaming probe: pure reads, the level-1 FP64 residual's (40 B read,
-# 4 B written per cell), 9:1, the finest line relaxation's (24 B read, 4 B written) and a copy
-STREAM_MIXES = ((10, 0), (10, 1), (9, 1), (6, 1), (1, 1))
+# 4 B written per cell), 9:1, the finest line relaxation's (24 B read, 4 B written) and a copy;
+# the last three again with every load and store carrying the non-temporal hint, as the finest
+# line kernels' once-read streams do on a level larger than the Infinity Cache
+STREAM_MIXES = ((10, 0, 0), (10, 1, 0), (9, 1, 0), (6, 1, 0), (1, 1, 0),
+                (9, 1, 1), (6, 1, 1), (1, 1, 1))
 
 
 def stream_ceiling(reps=5):
     """What plain streaming kernels reach on this GPU, as the practical ceilings next to the
     published 8 TB/s: fortranutils_amd/libstreamprobe.so reads R and writes W arrays of 16-byte
     elements (512 MiB each) in one grid-stride pass, best of ``reps`` timed launches, for each
-    mix of STREAM_MIXES.  HBM3E loses bandwidth when a read stream carries a small share of
-    writes, so a kernel is compared with the ceiling of its own read/write mix."""
+    mix of STREAM_MIXES, with ordinary or non-temporal accesses.  HBM3E loses bandwidth when a
+    read stream carries a small share of writes, so a kernel is compared with the ceiling of its
+    own read/write mix and access kind."""
     import ctypes as C
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fortranutils_amd",
                         "libstreamprobe.so")
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is not built: python -c 'import __graft_entry__ as g; g.build()'")
     lib = C.CDLL(path)
-    lib.lmgx_stream_probe.argtypes = [C.c_int, C.c_int, C.c_long, C.c_int,
-                                      C.POINTER(C.c_double)]
+    lib.lmgx_stream_probe_hint.argtypes = [C.c_int, C.c_int, C.c_int, C.c_long, C.c_int,
+                                           C.POINTER(C.c_double)]
     out = {}
-    for r, w in STREAM_MIXES:
+    for r, w, nt in STREAM_MIXES:
         v = C.c_double()
-        rc = lib.lmgx_stream_probe(r, w, 32 << 20, reps, C.byref(v))
+        rc = lib.lmgx_stream_probe_hint(r, w, nt, 32 << 20, reps, C.byref(v))
         if rc:
-            raise RuntimeError(f"stream probe R{r} W{w} failed ({rc})")
-        out[f"r{r}w{w}_GBps"] = v.value
+            raise RuntimeError(f"stream probe R{r} W{w} NT{nt} failed ({rc})")
+        out[f"r{r}w{w}{'nt' if nt else ''}_GBps"] = v.value
     return out
 
 
@@ -654,12 +658,18 @@ def main():
             sc = stream_ceiling()
             roof["stream_ceiling"] = sc
             # the kernel's own mix: b, the other colour's x, the old x and the couplings read,
-            # x written (6:1 with binary16 couplings, 9:1 without)
-            mix = "r6w1_GBps" if h16 else "r9w1_GBps"
-            roof["frac_of_stream"] = roof["achieved_moved"] / sc[mix]
+            # x written (6:1 with binary16 couplings, 9:1 without), with the non-temporal hint
+            # where the instance streams past the caches (all but the other colour's x)
+            mix = "r6w1" if h16 else "r9w1"
+            roof["frac_of_stream"] = roof["achieved_moved"] / sc[mix + ("nt" if fine == 1 else "")
+                                                                + "_GBps"]
+            roof["frac_of_stream_plain"] = roof["achieved_moved"] / sc[mix + "_GBps"]
+            roof["streamed_nontemporal"] = fine == 1
             print("# streaming on this GPU (reads:writes): " +
-                  ", ".join(f"{k[1:-5].replace('w', ':')} {v:.0f} GB/s" for k, v in sc.items()) +
-                  f"; smoother's moved bytes at {roof['frac_of_stream']:.2f} of its own mix's",
+                  ", ".join(f"{k[1:-5].replace('w', ':').replace('nt', ' nt')} {v:.0f} GB/s"
+                            for k, v in sc.items()) +
+                  f"; smoother's moved bytes at {roof['frac_of_stream']:.2f} of its own mix's"
+                  f" ({roof['frac_of_stream_plain']:.2f} of ordinary accesses')",
                   file=sys.stderr, flush=True)
 
     cpu = None

@@ -274,6 +274,13 @@ static int build_tables(lmgx_handle *h, const lmgx_topology *t)
                 B.sg_n = ns;
                 B.sg_r = (nj + ns - 1) / ns;
                 B.sg_s = 16 / ns;
+                // the over-relaxed solves on binary16 couplings and in FP64 run the body
+                // instance with the fewest rows that holds a segment (10, 12, 14 or 16: fewer
+                // repeated rows); the others and LMGX_SEG_FULL=1 take the 16-row one, which
+                // gives the same values
+                const char *full = getenv("LMGX_SEG_FULL");
+                B.sg_v = (full && atoi(full)) ? LMGX_SMAX
+                         : B.sg_r <= 10 ? 10 : B.sg_r <= 12 ? 12 : B.sg_r <= 14 ? 14 : LMGX_SMAX;
             }
             B.t_sg = tsg;
             tsg += (long)((B.hp + 63) / 64) * ((n[2] - 2 + B.sg_s - 1) / B.sg_s);

@@ -47,6 +47,8 @@ struct LevBlk {
     int sg_s;        // line sets (k-planes) per workgroup = 16 / sg_n
     long t_sg8;      // the same tiling for 8-wavefront workgroups (valid where sg_n <= 8)
     int sg_s8;       // line sets per 8-wavefront workgroup = 8 / sg_n (0 when sg_n = 16)
+    int sg_v;        // rows of the segmented body instance that runs the block: sg_r rounded up
+                     // to 10, 12, 14 or 16 (16 with LMGX_SEG_FULL=1)
     long s_jk;       // first (j,k) interior line of this block (hyperplane sweep)
     long o2;         // offset of the block's 2-D (i,j) metric arrays
     long o1;         // offset of the block's 1-D k metric arrays

@@ -607,6 +607,35 @@ def test_eight_wavefront_segmented_solve_is_the_same_solve(cfg, scale, monkeypat
             assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("kernel,prec", [("seg", 1), ("seg8", 1), ("seg8", 0)])
+@pytest.mark.parametrize("cfg,scale", [("c3", 4), ("c1", 2)])
+def test_short_segment_bodies_are_the_same_solve(cfg, scale, kernel, prec, monkeypatch):
+    """The over-relaxed segmented line solve on binary16 couplings (mixed precision) and in FP64
+    runs a block whose segments hold 10, 12 or 14 rows on a body instance of that many rows (c3
+    at 1/4 scale has lines of 16 to 64 rows on level 1: segments of 10, 12, 14 and 16 rows, and
+    of 8 on level 2); LMGX_SEG_FULL=1 runs every block on the 16-row body.  Rows past a segment
+    only repeat its last row, so both give bitwise the same pressure, residual and history, with
+    and without the zero-start half-sweeps, in 16- and 8-wavefront workgroups."""
+    import torch
+    monkeypatch.setenv("LMGX_LINE_KERNEL", kernel)
+    out = []
+    for full in ("1", "0"):
+        monkeypatch.setenv("LMGX_SEG_FULL", full)
+        p, o, s, coeffs, b = setup(cfg, scale)
+        for numpre in (1, 0):
+            f = torch.zeros(o.ntot, dtype=torch.float64, device="cuda")
+            res = torch.zeros_like(f)
+            hist = s.lmg_solve(f, b, res, max_cycles=4, tolf=0.0, rtol=0.0, smoother=1,
+                               precision=prec, numpre=numpre, numpost=1, omega=1.35,
+                               omega_c=1.0, sweeps_per_iter=3)
+            out.append((f.cpu().numpy(), hist, res.cpu().numpy()))
+        assert bool(s.coef16_levels()[0]) == (prec == 1)
+        assert (s.tile_widths()[prec][:-1] == (16 if kernel == "seg" else 8)).all()
+    for a, c in zip(out[:2], out[2:]):
+        for x, y in zip(a, c):
+            assert np.array_equal(x, y)
+
+
 @pytest.mark.parametrize("kernel", ["seg", "seg8"])
 @pytest.mark.parametrize("cfg,scale", [("c1", 4), ("c3", 4), ("c0", 1), ("c4", 4), ("c2", 8)])
 def test_metric_form_converges_like_stored_couplings(cfg, scale, kernel, monkeypatch):
