@@ -413,6 +413,23 @@ def cpu_baseline(args):
                 ranks=1, host_cores=host_cores(), kind="port", sample=sample, seconds=dt)
 
 
+def seg_dispatch(p):
+    """Whether the library runs level 1's segmented line solve with its short-segment bodies:
+    some block's lines cut into segments of at most 14 rows (the fewest power-of-two segments
+    of at most 16 rows), unless LMGX_SEG_FULL is set."""
+    full = os.environ.get("LMGX_SEG_FULL", "")
+    if full and int(full) != 0:
+        return False
+    for blk in p.blocks:
+        nj = blk.nijkp1[1] - 2
+        ns = 1
+        while ns < 16 and (nj + ns - 1) // ns > 16:
+            ns *= 2
+        if (nj + ns - 1) // ns <= 14:
+            return True
+    return False
+
+
 def traffic_from_profiles(workload, kernel):
     """HBM bytes per launch of ``kernel`` on ``workload`` from profiles/traffic.json (written by
     --traffic from rocprofv3 FETCH_SIZE / WRITE_SIZE runs); None when not measured."""
@@ -636,9 +653,11 @@ def main():
         # Infinity Cache (streamed past the caches), else 2 (the library's rule, LMGX_STREAM_MIB)
         lim = float(os.environ.get("LMGX_STREAM_MIB", "256")) * 1048576.0
         fine = 1 if 0.5 * cells_launch * 2 * (28 if word == 4 else 104) > lim else 2
-        # (the segmented kernels' last argument: level 1 in metric form, off by default)
-        key = (f"{segk}<{fp}, {fine}, {rl}, 0, {'true' if h16 else 'false'}, false>" if segk
-               else f"k_zebra_line<{fp}, {fine}, {tm}, {rl}, {fz}>")
+        # (the segmented kernels' last arguments: level 1 in metric form, off by default; the
+        # short-segment bodies, chosen per block, where some block's segments hold <= 14 rows)
+        dsp = "true" if (h16 or word == 8) and rl == "true" and seg_dispatch(p) else "false"
+        key = (f"{segk}<{fp}, {fine}, {rl}, 0, {'true' if h16 else 'false'}, false, {dsp}>"
+               if segk else f"k_zebra_line<{fp}, {fine}, {tm}, {rl}, {fz}>")
         tr = traffic_from_profiles(f"{args.config}:{args.mode}", key)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s",
                     frac=achieved / PEAK_HBM, traffic=tr,
