@@ -11,6 +11,9 @@
 #define LMGX_RCH 1      // coarse rows per thread of the residual column kernel
 #endif
 #define LMGX_RCHP 8     // coarse rows per thread of the prolongation column kernel
+#ifndef LMGX_CONV_NT
+#define LMGX_CONV_NT 1
+#endif
 #define LMGX_SMAX 16    // segmented line solve: rows per segment held in registers, 16 segments
                         // at most, so lines up to 256 rows
 
